@@ -1,0 +1,202 @@
+/*
+ * qcs_hip.h -- C ABI of the MI355X (gfx950) state-vector backend.
+ *
+ * Plain C89-compatible declarations: opaque handle, ints, longs, doubles and
+ * pointers only.  Every function returns 0 on success or a negative
+ * QCS_HIP_E* code; qcs_hip_last_error() gives the message of the most recent
+ * failure on the calling thread.  Nothing here falls back to host loops: when
+ * the device is missing the call fails with QCS_HIP_ENODEV.
+ *
+ * Each entry point replaces one host-side seam of the reference library
+ * (SegusFaultise/qcs); the reference file:line it replaces is cited on it.
+ *
+ * Indices are `long` (64-bit on LP64), so states up to 2^34 amplitudes work.
+ * Amplitudes cross the boundary as interleaved {re, im} doubles, the layout of
+ * the reference's `struct t_complex` (src/internal.h:6-9).
+ * A 2x2 gate crosses as 8 doubles, row-major {m00re, m00im, m01re, m01im,
+ * m10re, m10im, m11re, m11im}: the `data[0..3]` order of the reference's
+ * `struct t_q_matrix` (src/internal.h:55-59).
+ */
+#ifndef QCS_HIP_H
+#define QCS_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct qcs_hip_state qcs_hip_state;
+
+/* precision of the device state */
+#define QCS_HIP_C128 0 /* complex128: {double re, im} per amplitude */
+#define QCS_HIP_C64 1  /* complex64:  {float re, im} per amplitude */
+
+/* state flags */
+#define QCS_HIP_COMPAT 1 /* keep the reference's double buffer (vector +
+                            scratch_vector, src/q_state.c:49-66) and apply the
+                            as-written controlled-gate / Grover maps */
+
+/* error codes */
+#define QCS_HIP_OK 0
+#define QCS_HIP_EINVAL -1 /* bad argument (qubit, size, pointer) */
+#define QCS_HIP_ENODEV -2 /* no HIP device / runtime failure at init */
+#define QCS_HIP_ENOMEM -3 /* device allocation failed */
+#define QCS_HIP_EDEVICE -4 /* a HIP call or kernel launch failed */
+#define QCS_HIP_EUNSUP -5 /* combination not supported (e.g. fusion in
+                             compat mode) */
+
+/* ---- device / error -------------------------------------------------- */
+const char *qcs_hip_last_error(void);
+/* 1 if a HIP device is usable, 0 otherwise (never aborts) */
+int qcs_hip_device_available(void);
+/* hipDeviceSynchronize on the current device */
+int qcs_hip_device_sync(void);
+
+/* ---- state lifetime ---------------------------------------------------
+ * Replaces q_state_init / q_state_free (src/q_state.c:32-115): allocates
+ * 2^num_qubits amplitudes in HBM (a second buffer only with QCS_HIP_COMPAT),
+ * sets |0...0>, binds one HIP stream to the state.  Unlike the reference's
+ * OpenCL backend (process-global buffers, src/gpu_complex.c:6-20) every state
+ * owns its device memory. */
+int qcs_hip_state_create(qcs_hip_state **out, int num_qubits, int precision,
+                         int flags);
+void qcs_hip_state_destroy(qcs_hip_state *s);
+int qcs_hip_state_num_qubits(const qcs_hip_state *s);
+int qcs_hip_state_precision(const qcs_hip_state *s);
+int qcs_hip_state_flags(const qcs_hip_state *s);
+/* |basis>; the compat scratch buffer is zeroed as q_state_init does
+ * (src/q_state.c:93-99) */
+int qcs_hip_state_set_basis(qcs_hip_state *s, long basis);
+/* wait for every op queued on the state's stream */
+int qcs_hip_sync(qcs_hip_state *s);
+
+/* ---- host <-> device copies (observation, tests) -----------------------
+ * `out` / `in` hold 2*count doubles {re, im}; c64 states convert. */
+int qcs_hip_read(qcs_hip_state *s, long offset, long count, double *out);
+int qcs_hip_write(qcs_hip_state *s, long offset, long count, const double *in);
+/* compat mode only: the scratch buffer (reference `scratch_vector`) */
+int qcs_hip_read_scratch(qcs_hip_state *s, long offset, long count,
+                         double *out);
+
+/* ---- gates -------------------------------------------------------------
+ * qcs_hip_apply_1q replaces q_apply_1q_gate / q_apply_1q_gate_gpu
+ * (src/q_gates.c:25-149, src/gpu_complex.c:521-695): one in-place pair sweep
+ * (out-of-place plus buffer swap in compat mode).  Arithmetic per output is
+ * c_add(c_mul(g0, v0), c_mul(g1, v1)) without FMA contraction
+ * (src/complex.c:23-57), so c128 results equal the reference's sequential
+ * loop bit for bit. */
+int qcs_hip_apply_1q(qcs_hip_state *s, int target, const double m[8]);
+/* qcs_hip_apply_cu replaces q_apply_2q_gate / q_apply_2q_gate_gpu
+ * (src/q_gates.c:158-298, src/gpu_complex.c:704-921): controlled 2x2 on
+ * `target` where bit `control` is set.  Textbook mode touches only the
+ * control=1 half (CNOT: swap, CP: the c=1,t=1 quarter).  Compat mode
+ * reproduces the as-written sequential map out[i0] = m00*v0 + m01*v1,
+ * out[i1] = v1 (src/q_gates.c:279-292). */
+int qcs_hip_apply_cu(qcs_hip_state *s, int control, int target,
+                     const double m[8]);
+/* SWAP of two qubits (no reference counterpart; used for QFT final swaps).
+ * Textbook mode only. */
+int qcs_hip_apply_swap(qcs_hip_state *s, int qubit_a, int qubit_b);
+
+/* ---- Grover ------------------------------------------------------------
+ * Replace q_apply_phase_flip (src/q_gates.c:305-317) and q_apply_diffusion
+ * (src/q_gates.c:323-356).  Textbook: a[m] = -a[m]; a = 2*(sum a)/N - a.
+ * Compat: the as-written buffer swaps and unit-magnitude mean. */
+int qcs_hip_phase_flip(qcs_hip_state *s, long index);
+int qcs_hip_diffusion(qcs_hip_state *s);
+/* `iterations` rounds of (phase_flip(marked); diffusion) on the device with
+ * no host round trip (textbook: one fused read+write pass per round).  The
+ * loop body of qc_grover_search (src/qcs.c:412-418). */
+int qcs_hip_grover(qcs_hip_state *s, long marked, long iterations);
+
+/* ---- reductions and observation (FP64 on the device) -------------------
+ * norm_sq = sum |a|^2 (q_state_normalize's sum, src/q_utils.c:102-110) */
+int qcs_hip_norm_sq(qcs_hip_state *s, double *norm_sq);
+/* q_state_normalize (src/q_utils.c:102-118): if norm_sq > 1e-12 and
+ * norm_sq != 1.0 scale by 1/sqrt(norm_sq).  `norm_sq_out` may be NULL. */
+int qcs_hip_normalize(qcs_hip_state *s, double *norm_sq_out);
+/* sums of |a|^2 over bit `qubit` = 0 and = 1 (qc_measure's p0 sweep,
+ * src/qcs.c:253-259) */
+int qcs_hip_qubit_probs(qcs_hip_state *s, int qubit, double *p0, double *p1);
+/* qc_measure's collapse (src/qcs.c:261-283): with u = rand()/RAND_MAX
+ * supplied by the host, outcome = (u <= p0) ? 0 : 1; zero the other half and
+ * normalize.  Returns the outcome in *outcome. */
+int qcs_hip_measure(qcs_hip_state *s, int qubit, double u, int *outcome);
+/* |a_index|^2 computed as re*re + im*im (qc_get_probability,
+ * src/qcs.c:391-395) */
+int qcs_hip_probability(qcs_hip_state *s, long index, double *p);
+/* first index of the largest |a|^2; 0 when all are zero
+ * (qc_find_most_likely_state, src/qcs.c:464-478) */
+int qcs_hip_argmax(qcs_hip_state *s, long *index);
+/* qc_run_shots (src/qcs.c:575-607): for each uniform u[s] (host rand()
+ * stream) the first index i with u < sum_{j<=i} |a_j|^2, or -1 when u is not
+ * below the total (the reference drops such shots).  FP64 chunk sums, a
+ * prefix scan of the chunk sums and a per-shot search; the CDF is never
+ * materialised. */
+int qcs_hip_sample(qcs_hip_state *s, long shots, const double *u,
+                   long *index_out);
+
+/* ---- fused precision form ----------------------------------------------
+ * A gate for the fusion pipeline (textbook semantics). */
+#define QCS_HIP_G_1Q 0   /* 2x2 `m` on `target` */
+#define QCS_HIP_G_CU 1   /* 2x2 `m` on `target` where bit `control` = 1 */
+#define QCS_HIP_G_SWAP 2 /* swap qubits `target` and `control` */
+typedef struct qcs_hip_gate {
+  int kind;
+  int target;
+  int control;
+  int reserved;
+  double m[8];
+} qcs_hip_gate;
+
+/* One fused block: `num_gates` consecutive entries of the gate array,
+ * starting at `first_gate`, acting only on the k qubits `targets`
+ * (ascending, 1 <= k <= 5). */
+#define QCS_HIP_MAX_FUSE 5
+typedef struct qcs_hip_block {
+  int k;
+  int first_gate;
+  int num_gates;
+  int reserved;
+  int targets[8];
+} qcs_hip_block;
+
+/* Compose each block's 2^k x 2^k unitary on the device in FP64, round it to
+ * FP32 and apply it to the state: c64 states as a complex GEMM on the FP32
+ * matrix cores (four real products, FP32 accumulation); c128 states with an
+ * FP64 dense kernel.  Textbook mode only.  No reference counterpart (the
+ * reference applies every gate eagerly, src/qcs.c:159-164). */
+int qcs_hip_apply_fused(qcs_hip_state *s, const qcs_hip_block *blocks,
+                        int num_blocks, const qcs_hip_gate *gates,
+                        int num_gates);
+/* Apply one dense k-qubit unitary U (2^k x 2^k complex, row-major,
+ * interleaved {re, im} doubles) on ascending `targets`. */
+int qcs_hip_apply_unitary(qcs_hip_state *s, int k, const int *targets,
+                          const double *u);
+
+/* ---- comparison ----------------------------------------------------------
+ * FP64 on the device: one_minus_f = 1 - |<a|b>|^2 / (|a|^2 |b|^2) and the
+ * max-abs amplitude difference.  States may differ in precision but not in
+ * size.  Either output may be NULL. */
+int qcs_hip_compare(qcs_hip_state *a, qcs_hip_state *b, double *one_minus_f,
+                    double *max_abs);
+
+/* ---- kernel timing ----------------------------------------------------
+ * When enabled, every kernel the state launches is bracketed by HIP events
+ * recorded on the state's stream; totals are kept per kernel class. */
+#define QCS_HIP_K_1Q 0
+#define QCS_HIP_K_CU 1
+#define QCS_HIP_K_FUSED 2
+#define QCS_HIP_K_COMPOSE 3
+#define QCS_HIP_K_REDUCE 4
+#define QCS_HIP_K_OTHER 5
+#define QCS_HIP_K_COUNT 6
+int qcs_hip_timing_enable(qcs_hip_state *s, int enable);
+/* per class: total ms, launches, algorithmic bytes, flops since enable */
+int qcs_hip_timing_read(qcs_hip_state *s, int kernel_class, double *total_ms,
+                        long *launches, double *bytes, double *flops);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QCS_HIP_H */

@@ -1,0 +1,86 @@
+// Device helpers shared by the kernels: amplitude types, index-bit insertion
+// and the reference's complex arithmetic order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace qcs {
+
+template <typename T>
+struct Amp;
+template <>
+struct Amp<double> {
+  typedef double2 type;
+};
+template <>
+struct Amp<float> {
+  typedef float2 type;
+};
+
+// A 2x2 gate in the precision of the state: {m00, m01, m10, m11} as re/im.
+template <typename T>
+struct Mat2 {
+  T v[8];
+};
+
+// Insert a zero bit at position b of x (pairs of a gate on qubit b).
+__device__ __forceinline__ uint64_t insert0(uint64_t x, int b) {
+  const uint64_t lo = x & ((uint64_t(1) << b) - 1);
+  return ((x >> b) << (b + 1)) | lo;
+}
+
+// c_mul of the reference (src/complex.c:47-57):
+//   re = ar*br - ai*bi, im = ar*bi + ai*br
+// The library is built with -ffp-contract=off so these stay separate
+// multiplies and adds, as gcc -std=c89 compiles the reference.
+template <typename T>
+__device__ __forceinline__ void cmul(T ar, T ai, T br, T bi, T &rr, T &ri) {
+  rr = ar * br - ai * bi;
+  ri = ar * bi + ai * br;
+}
+
+// out = c_add(c_mul(g0, v0), c_mul(g1, v1))   (src/q_gates.c:140-141)
+template <typename T>
+__device__ __forceinline__ void row2(T g0r, T g0i, T g1r, T g1i, T v0r, T v0i,
+                                     T v1r, T v1i, T &orr, T &oi) {
+  T ar, ai, br, bi;
+  cmul(g0r, g0i, v0r, v0i, ar, ai);
+  cmul(g1r, g1i, v1r, v1i, br, bi);
+  orr = ar + br;
+  oi = ai + bi;
+}
+
+// |a|^2 = re*re + im*im in FP64 (c_norm_sq, src/complex.c:89-93)
+template <typename A>
+__device__ __forceinline__ double norm2(const A &a) {
+  const double re = a.x, im = a.y;
+  return re * re + im * im;
+}
+
+// Wave-level (64 lanes) and block-level FP64 sums with a fixed association,
+// so every reduction is deterministic run to run.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Sum of v over the work-group; `lds` holds blockDim.x/64 doubles.
+// The result is valid in thread 0.
+__device__ __forceinline__ double block_sum(double v, double *lds) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) lds[wave] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) {
+    const int nw = blockDim.x >> 6;
+    for (int w = 0; w < nw; ++w) r += lds[w];
+  }
+  __syncthreads();
+  return r;
+}
+
+}  // namespace qcs

@@ -1,0 +1,316 @@
+// Plain-form gate kernels: one HBM sweep per gate.
+//
+// 1-qubit gate on target t: pair p in [0, N/2) -> i0 = p with a 0 inserted at
+// bit t, i1 = i0 | 2^t (the reference's block/step loop, src/q_gates.c:133-143,
+// flattened).  Consecutive threads take consecutive pairs, so for t >= 6 both
+// halves of a wave's pairs are single 1 KiB runs (c128); for low t the two
+// 16-byte loads of a thread sit in the same cache lines and L2 serves the
+// second touch, so HBM still moves each byte once.
+//
+// Controlled gates select amplitudes by index-bit masks: index q in
+// [0, N/4) gets zeros inserted at the control and target bits, then the
+// control bit is set; only the control=1 half is read and written.
+#include "device.h"
+#include "internal.h"
+
+namespace qcs {
+
+constexpr int kUnroll = 2;  // pairs per thread: two independent loads in flight
+
+// ---- 1-qubit gates --------------------------------------------------------
+
+// Dense 2x2 on (a[i0], a[i1]); `in` may equal `out` (in place).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_1q_dense(
+    const typename Amp<T>::type *in, typename Amp<T>::type *out,
+    uint64_t npairs, int t, Mat2<T> m) {
+  typedef typename Amp<T>::type A;
+  const uint64_t tb = uint64_t(1) << t;
+  const uint64_t base = uint64_t(blockIdx.x) * (kThreads * kUnroll) + threadIdx.x;
+  A v0[kUnroll], v1[kUnroll];
+  uint64_t i0[kUnroll];
+#pragma unroll
+  for (int u = 0; u < kUnroll; ++u) {
+    const uint64_t p = base + uint64_t(u) * kThreads;
+    i0[u] = insert0(p, t);
+    if (p < npairs) {
+      v0[u] = in[i0[u]];
+      v1[u] = in[i0[u] | tb];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < kUnroll; ++u) {
+    const uint64_t p = base + uint64_t(u) * kThreads;
+    if (p >= npairs) continue;
+    A o0, o1;
+    row2(m.v[0], m.v[1], m.v[2], m.v[3], v0[u].x, v0[u].y, v1[u].x, v1[u].y,
+         o0.x, o0.y);
+    row2(m.v[4], m.v[5], m.v[6], m.v[7], v0[u].x, v0[u].y, v1[u].x, v1[u].y,
+         o1.x, o1.y);
+    out[i0[u]] = o0;
+    out[i0[u] | tb] = o1;
+  }
+}
+
+// diag(1, m11) on target t (Z, P): only the bit-t=1 half changes.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_1q_phase(
+    typename Amp<T>::type *a, uint64_t nhalf, int t, T mr, T mi) {
+  typedef typename Amp<T>::type A;
+  const uint64_t tb = uint64_t(1) << t;
+  const uint64_t base = uint64_t(blockIdx.x) * (kThreads * kUnroll) + threadIdx.x;
+  A v[kUnroll];
+  uint64_t idx[kUnroll];
+#pragma unroll
+  for (int u = 0; u < kUnroll; ++u) {
+    const uint64_t p = base + uint64_t(u) * kThreads;
+    idx[u] = insert0(p, t) | tb;
+    if (p < nhalf) v[u] = a[idx[u]];
+  }
+#pragma unroll
+  for (int u = 0; u < kUnroll; ++u) {
+    const uint64_t p = base + uint64_t(u) * kThreads;
+    if (p >= nhalf) continue;
+    A o;
+    cmul(mr, mi, v[u].x, v[u].y, o.x, o.y);
+    a[idx[u]] = o;
+  }
+}
+
+// ---- controlled gates (textbook) --------------------------------------------
+
+// index of quarter q with zeros inserted at bits lo < hi, then `set` OR-ed in
+template <int KIND, typename T>
+__global__ __launch_bounds__(kThreads) void k_cu(typename Amp<T>::type *a,
+                                                 uint64_t nquarter, int lo,
+                                                 int hi, uint64_t setb,
+                                                 uint64_t tb, Mat2<T> m) {
+  // KIND 0: dense 2x2 on (i0, i1); 1: swap (CNOT); 2: phase on i1 only (CP)
+  typedef typename Amp<T>::type A;
+  const uint64_t base = uint64_t(blockIdx.x) * (kThreads * kUnroll) + threadIdx.x;
+  A v0[kUnroll], v1[kUnroll];
+  uint64_t i0[kUnroll];
+#pragma unroll
+  for (int u = 0; u < kUnroll; ++u) {
+    const uint64_t q = base + uint64_t(u) * kThreads;
+    i0[u] = insert0(insert0(q, lo), hi) | setb;
+    if (q < nquarter) {
+      if (KIND != 2) v0[u] = a[i0[u]];
+      v1[u] = a[i0[u] | tb];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < kUnroll; ++u) {
+    const uint64_t q = base + uint64_t(u) * kThreads;
+    if (q >= nquarter) continue;
+    if (KIND == 0) {
+      A o0, o1;
+      row2(m.v[0], m.v[1], m.v[2], m.v[3], v0[u].x, v0[u].y, v1[u].x, v1[u].y,
+           o0.x, o0.y);
+      row2(m.v[4], m.v[5], m.v[6], m.v[7], v0[u].x, v0[u].y, v1[u].x, v1[u].y,
+           o1.x, o1.y);
+      a[i0[u]] = o0;
+      a[i0[u] | tb] = o1;
+    } else if (KIND == 1) {
+      a[i0[u]] = v1[u];
+      a[i0[u] | tb] = v0[u];
+    } else {
+      A o;
+      cmul(m.v[6], m.v[7], v1[u].x, v1[u].y, o.x, o.y);
+      a[i0[u] | tb] = o;
+    }
+  }
+}
+
+// Compat controlled gate, out of place over all N (src/q_gates.c:277-292 as
+// written): out[i] = m00*v[i] + m01*v[i|t] where bit c = 1 and bit t = 0,
+// out[i] = v[i] everywhere else (iteration i|t overwrites the partner).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_cu_compat(
+    const typename Amp<T>::type *in, typename Amp<T>::type *out, uint64_t n,
+    uint64_t cb, uint64_t tb, Mat2<T> m) {
+  typedef typename Amp<T>::type A;
+  const uint64_t base = uint64_t(blockIdx.x) * (kThreads * kUnroll) + threadIdx.x;
+#pragma unroll
+  for (int u = 0; u < kUnroll; ++u) {
+    const uint64_t i = base + uint64_t(u) * kThreads;
+    if (i >= n) continue;
+    const A v = in[i];
+    if ((i & cb) && !(i & tb)) {
+      const A w = in[i | tb];
+      A o;
+      row2(m.v[0], m.v[1], m.v[2], m.v[3], v.x, v.y, w.x, w.y, o.x, o.y);
+      out[i] = o;
+    } else {
+      out[i] = v;
+    }
+  }
+}
+
+// SWAP of qubits a < b: exchange amplitudes with (bit a, bit b) = (1,0), (0,1)
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_swapq(typename Amp<T>::type *a,
+                                                    uint64_t nquarter, int lo,
+                                                    int hi) {
+  typedef typename Amp<T>::type A;
+  const uint64_t bl = uint64_t(1) << lo, bh = uint64_t(1) << hi;
+  const uint64_t base = uint64_t(blockIdx.x) * (kThreads * kUnroll) + threadIdx.x;
+#pragma unroll
+  for (int u = 0; u < kUnroll; ++u) {
+    const uint64_t q = base + uint64_t(u) * kThreads;
+    if (q >= nquarter) continue;
+    const uint64_t i = insert0(insert0(q, lo), hi);
+    const A x = a[i | bl];
+    const A y = a[i | bh];
+    a[i | bl] = y;
+    a[i | bh] = x;
+  }
+}
+
+// Fill with zeros and optionally set a[basis] = 1.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_fill(typename Amp<T>::type *a,
+                                                   uint64_t n, uint64_t basis,
+                                                   int one) {
+  typedef typename Amp<T>::type A;
+  const uint64_t stride = uint64_t(gridDim.x) * kThreads;
+  for (uint64_t i = uint64_t(blockIdx.x) * kThreads + threadIdx.x; i < n;
+       i += stride) {
+    A v;
+    v.x = (one && i == basis) ? T(1) : T(0);
+    v.y = T(0);
+    a[i] = v;
+  }
+}
+
+// ---- host launchers ------------------------------------------------------------
+
+template <typename T>
+static Mat2<T> to_mat(const double m[8]) {
+  Mat2<T> r;
+  for (int i = 0; i < 8; ++i) r.v[i] = T(m[i]);
+  return r;
+}
+
+static bool is_phase1(const double m[8]) {
+  // diag(1, x): m00 = 1, m01 = m10 = 0
+  return m[0] == 1.0 && m[1] == 0.0 && m[2] == 0.0 && m[3] == 0.0 &&
+         m[4] == 0.0 && m[5] == 0.0;
+}
+
+static bool is_x(const double m[8]) {
+  return m[0] == 0.0 && m[1] == 0.0 && m[2] == 1.0 && m[3] == 0.0 &&
+         m[4] == 1.0 && m[5] == 0.0 && m[6] == 0.0 && m[7] == 0.0;
+}
+
+static unsigned grid_for(uint64_t items) {
+  return unsigned(ceil_div(items, uint64_t(kThreads) * kUnroll));
+}
+
+template <typename T>
+static int do_1q(qcs_hip_state *s, int t, const double md[8]) {
+  typedef typename Amp<T>::type A;
+  const Mat2<T> m = to_mat<T>(md);
+  const bool compat = (s->flags & QCS_HIP_COMPAT) != 0;
+  A *cur = static_cast<A *>(s->vec);
+  const double N = double(s->size), sb = double(sizeof(A));
+  TimedLaunch tl;
+  if (!compat && is_phase1(md)) {
+    const uint64_t nhalf = s->size / 2;
+    timing_begin(s, QCS_HIP_K_1Q, N * sb, 3.0 * N, &tl);
+    hipLaunchKernelGGL(k_1q_phase<T>, dim3(grid_for(nhalf)), dim3(kThreads), 0,
+                       s->stream, cur, nhalf, t, m.v[6], m.v[7]);
+  } else {
+    A *out = compat ? static_cast<A *>(s->scratch) : cur;
+    const uint64_t npairs = s->size / 2;
+    timing_begin(s, QCS_HIP_K_1Q, 2.0 * N * sb, 14.0 * N, &tl);
+    hipLaunchKernelGGL(k_1q_dense<T>, dim3(grid_for(npairs)), dim3(kThreads), 0,
+                       s->stream, cur, out, npairs, t, m);
+    if (compat) {  // src/q_gates.c:146-148
+      s->scratch = s->vec;
+      s->vec = out;
+    }
+  }
+  int rc = launch_check("k_1q");
+  if (rc) return rc;
+  return timing_end(s, &tl);
+}
+
+template <typename T>
+static int do_cu(qcs_hip_state *s, int c, int t, const double md[8]) {
+  typedef typename Amp<T>::type A;
+  const Mat2<T> m = to_mat<T>(md);
+  A *cur = static_cast<A *>(s->vec);
+  const uint64_t cb = uint64_t(1) << c, tb = uint64_t(1) << t;
+  const double N = double(s->size), sb = double(sizeof(A));
+  TimedLaunch tl;
+  if (s->flags & QCS_HIP_COMPAT) {
+    A *out = static_cast<A *>(s->scratch);
+    timing_begin(s, QCS_HIP_K_CU, 2.0 * N * sb, 0.0, &tl);
+    hipLaunchKernelGGL(k_cu_compat<T>, dim3(grid_for(s->size)), dim3(kThreads),
+                       0, s->stream, cur, out, s->size, cb, tb, m);
+    s->scratch = s->vec;
+    s->vec = out;
+  } else {
+    const int lo = c < t ? c : t, hi = c < t ? t : c;
+    const uint64_t nq = s->size / 4;
+    const dim3 g(grid_for(nq)), b(kThreads);
+    if (is_x(md)) {
+      timing_begin(s, QCS_HIP_K_CU, N * sb, 0.0, &tl);
+      hipLaunchKernelGGL((k_cu<1, T>), g, b, 0, s->stream, cur, nq, lo, hi, cb,
+                         tb, m);
+    } else if (is_phase1(md)) {
+      timing_begin(s, QCS_HIP_K_CU, 0.5 * N * sb, 1.5 * N, &tl);
+      hipLaunchKernelGGL((k_cu<2, T>), g, b, 0, s->stream, cur, nq, lo, hi,
+                         cb | tb, uint64_t(0), m);
+    } else {
+      timing_begin(s, QCS_HIP_K_CU, N * sb, 7.0 * N, &tl);
+      hipLaunchKernelGGL((k_cu<0, T>), g, b, 0, s->stream, cur, nq, lo, hi, cb,
+                         tb, m);
+    }
+  }
+  int rc = launch_check("k_cu");
+  if (rc) return rc;
+  return timing_end(s, &tl);
+}
+
+int launch_1q(qcs_hip_state *s, int target, const double m[8]) {
+  return s->precision == QCS_HIP_C128 ? do_1q<double>(s, target, m)
+                                      : do_1q<float>(s, target, m);
+}
+
+int launch_cu(qcs_hip_state *s, int control, int target, const double m[8]) {
+  return s->precision == QCS_HIP_C128 ? do_cu<double>(s, control, target, m)
+                                      : do_cu<float>(s, control, target, m);
+}
+
+int launch_swap(qcs_hip_state *s, int a, int b) {
+  const int lo = a < b ? a : b, hi = a < b ? b : a;
+  const uint64_t nq = s->size / 4;
+  TimedLaunch tl;
+  timing_begin(s, QCS_HIP_K_CU, double(s->size) * double(s->amp_bytes), 0.0,
+               &tl);
+  if (s->precision == QCS_HIP_C128)
+    hipLaunchKernelGGL(k_swapq<double>, dim3(grid_for(nq)), dim3(kThreads), 0,
+                       s->stream, static_cast<double2 *>(s->vec), nq, lo, hi);
+  else
+    hipLaunchKernelGGL(k_swapq<float>, dim3(grid_for(nq)), dim3(kThreads), 0,
+                       s->stream, static_cast<float2 *>(s->vec), nq, lo, hi);
+  int rc = launch_check("k_swapq");
+  if (rc) return rc;
+  return timing_end(s, &tl);
+}
+
+int launch_fill_basis(qcs_hip_state *s, void *buf, uint64_t basis, int one) {
+  const uint64_t blocks = ceil_div(s->size, kThreads);
+  const unsigned g = unsigned(blocks < 65536 ? blocks : 65536);
+  if (s->precision == QCS_HIP_C128)
+    hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(kThreads), 0, s->stream,
+                       static_cast<double2 *>(buf), s->size, basis, one);
+  else
+    hipLaunchKernelGGL(k_fill<float>, dim3(g), dim3(kThreads), 0, s->stream,
+                       static_cast<float2 *>(buf), s->size, basis, one);
+  return launch_check("k_fill");
+}
+
+}  // namespace qcs

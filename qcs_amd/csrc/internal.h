@@ -1,0 +1,125 @@
+// Internal declarations of the gfx950 state-vector backend.
+//
+// Layout in HBM: one array of 2^n amplitudes, interleaved {re, im}
+// (double2 for complex128, float2 for complex64), index bit q = qubit q
+// (qubit 0 is the least significant bit, as `1L << qubit` in the reference,
+// src/q_gates.c:28).  Every update is in place; compat mode adds the
+// reference's second buffer and swaps the two pointers like
+// src/q_gates.c:146-148.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "../../include/qcs_hip.h"
+
+namespace qcs {
+
+constexpr int kThreads = 256;     // work-group size of the streaming kernels
+constexpr int kRedBlocks = 1024;  // fixed grid of the FP64 reductions
+constexpr int kRedSlots = 8;      // doubles per reduction partial
+constexpr long kChunk = 4096;     // amplitudes per sampling chunk
+constexpr int kMaxQubits = 34;
+
+struct TimedLaunch {
+  hipEvent_t start;
+  hipEvent_t stop;
+  int cls;
+  double bytes;
+  double flops;
+};
+
+}  // namespace qcs
+
+struct qcs_hip_state {
+  int n = 0;
+  int precision = QCS_HIP_C128;
+  int flags = 0;
+  int device = 0;
+  uint64_t size = 0;
+  size_t amp_bytes = 16;
+  void *vec = nullptr;      // current amplitudes
+  void *scratch = nullptr;  // compat: the reference's scratch_vector
+  hipStream_t stream = nullptr;
+  double *red = nullptr;    // kRedBlocks * kRedSlots partials + result slots
+  double *host = nullptr;   // pinned staging for small read-backs
+  size_t host_bytes = 0;
+  // fused-form device buffers (grown on demand)
+  void *gates_dev = nullptr;
+  size_t gates_cap = 0;
+  void *blocks_dev = nullptr;
+  size_t blocks_cap = 0;
+  float *wbuf = nullptr;    // per block: real 2^(k+1) x 2^(k+1) FP32 matrix
+  double *ubuf = nullptr;   // per block: complex 2^k x 2^k FP64 matrix
+  size_t wbuf_blocks = 0;
+  // sampling buffers
+  double *chunk_tot = nullptr;  // per chunk totals, then exclusive prefix
+  double *chunk_inc = nullptr;  // inclusive prefix per chunk
+  size_t chunk_cap = 0;
+  double *u_dev = nullptr;
+  long *idx_dev = nullptr;
+  size_t shots_cap = 0;
+  // kernel timing
+  int timing = 0;
+  std::vector<qcs::TimedLaunch> pending_events;
+  std::vector<hipEvent_t> free_events;
+  double t_ms[QCS_HIP_K_COUNT] = {0};
+  long t_launches[QCS_HIP_K_COUNT] = {0};
+  double t_bytes[QCS_HIP_K_COUNT] = {0};
+  double t_flops[QCS_HIP_K_COUNT] = {0};
+};
+
+namespace qcs {
+
+// error reporting (thread-local message)
+int fail(int code, const char *fmt, ...);
+int check_hip(hipError_t e, const char *what);
+#define QCS_HIP_TRY(expr)                              \
+  do {                                                 \
+    hipError_t qcs_e_ = (expr);                        \
+    if (qcs_e_ != hipSuccess) return qcs::check_hip(qcs_e_, #expr); \
+  } while (0)
+
+// launch bracketing for the kernel timers
+void timing_begin(qcs_hip_state *s, int cls, double bytes, double flops,
+                  TimedLaunch *tl);
+int timing_end(qcs_hip_state *s, TimedLaunch *tl);
+// after a launch: check for a launch error
+int launch_check(const char *what);
+
+// buffer helpers
+int ensure_pinned(qcs_hip_state *s, size_t bytes);
+
+// kernels in gates.hip
+int launch_1q(qcs_hip_state *s, int target, const double m[8]);
+int launch_cu(qcs_hip_state *s, int control, int target, const double m[8]);
+int launch_swap(qcs_hip_state *s, int a, int b);
+int launch_fill_basis(qcs_hip_state *s, void *buf, uint64_t basis, int one);
+
+// kernels in reduce.hip
+int reduce_norm(qcs_hip_state *s, double *out_host);  // host result
+int normalize_device(qcs_hip_state *s);               // no host sync
+int scale_state(qcs_hip_state *s, double factor);
+int qubit_probs(qcs_hip_state *s, int q, double *p0, double *p1);
+int collapse(qcs_hip_state *s, int q, int outcome, double factor, int do_scale);
+int argmax(qcs_hip_state *s, long *idx);
+int sample(qcs_hip_state *s, long shots, const double *u, long *out);
+int grover_textbook(qcs_hip_state *s, long marked, long iters);
+int phase_flip_compat(qcs_hip_state *s, long index);
+int diffusion_compat(qcs_hip_state *s);
+int negate_one(qcs_hip_state *s, long index);
+int compare(qcs_hip_state *a, qcs_hip_state *b, double *omf, double *maxabs);
+
+// kernels in fused.hip
+int fused_apply(qcs_hip_state *s, const qcs_hip_block *blocks, int nb,
+                const qcs_hip_gate *gates, int ng);
+int fused_apply_unitary(qcs_hip_state *s, int k, const int *targets,
+                        const double *u);
+
+__host__ __device__ inline uint64_t ceil_div(uint64_t a, uint64_t b) {
+  return (a + b - 1) / b;
+}
+
+}  // namespace qcs

@@ -1,0 +1,38 @@
+/* Internal declarations of the C89 host library. */
+#ifndef QCS_INTERNAL_H
+#define QCS_INTERNAL_H
+
+#include "qcs.h"
+#include "qcs_hip.h"
+
+struct t_q_circuit {
+  int num_qubits;
+  int num_gates;
+  qcs_hip_state *state;
+  int options;
+  int error;
+  /* gate history, same columns as the reference (src/qcs.c:13-23) */
+  char **gate_history;
+  int *target_qubits;
+  int *control_qubits;
+  double *parameters;
+  int history_size;
+  int history_capacity;
+  /* gates recorded for the fused form, applied at the next flush */
+  qcs_hip_gate *pending;
+  int pending_size;
+  int pending_capacity;
+  long fused_blocks;
+};
+
+/* Greedy fusion planner (fusion.c).
+ * Input: `ng` gates in circuit order on `n` qubits.  Output: `*nblocks`
+ * blocks of at most `kmax` qubits each and the gates re-ordered so that each
+ * block's gates are contiguous (block b owns ordered[first .. first+num)).
+ * Arrays are malloc'ed; the caller frees them.  Returns 0 or -1 (no memory /
+ * bad input). */
+int qcs_fusion_plan(const qcs_hip_gate *gates, int ng, int n, int kmax,
+                    qcs_hip_block **blocks, int *nblocks,
+                    qcs_hip_gate **ordered);
+
+#endif

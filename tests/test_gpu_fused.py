@@ -1,0 +1,161 @@
+"""Fused precision form on the GPU.
+
+* qcs_hip_apply_unitary: a random dense k-qubit unitary (k = 1..5) on a
+  complex64 state goes through the FP32 matrix-core kernel (n >= 12) or the
+  FP32 VALU kernel (n < 12); compared with numpy in FP64: max-abs <= 2e-6
+  on a unit-norm state.  complex128 states take the FP64 VALU kernel:
+  <= 1e-12.
+* Fused circuits vs the plain complex128 run of the same circuit on the same
+  GPU: 1 - F <= 1e-6 (the precision form's tolerance).
+"""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+import helpers as HP
+import qcs_amd
+from qcs_amd import circuits as C
+
+pytestmark = pytest.mark.gpu
+
+
+def rand_unitary(k, rng):
+    z = rng.normal(size=(1 << k, 1 << k)) + 1j * rng.normal(size=(1 << k, 1 << k))
+    q, r = np.linalg.qr(z)
+    return q * (np.diag(r) / np.abs(np.diag(r)))
+
+
+def rand_state(n, rng):
+    v = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    return v / np.linalg.norm(v)
+
+
+def apply_dense(psi, n, targets, U):
+    """numpy FP64 reference: U on ascending `targets` (targets[0] = U's LSB)."""
+    k = len(targets)
+    out = psi.copy()
+    idx = np.arange(1 << n)
+    mask = sum(1 << t for t in targets)
+    base = idx[(idx & mask) == 0]
+    offs = np.array([sum(((j >> q) & 1) << targets[q] for q in range(k))
+                     for j in range(1 << k)])
+    X = psi[base[None, :] + offs[:, None]]
+    out[base[None, :] + offs[:, None]] = U @ X
+    return out
+
+
+def hip_apply_unitary(hip, state, targets, U):
+    t = (ctypes.c_int * len(targets))(*targets)
+    u = np.ascontiguousarray(U, np.complex128).view(np.float64)
+    rc = hip.qcs_hip_apply_unitary(state, len(targets), t,
+                                   u.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    assert rc == 0, hip.qcs_hip_last_error()
+
+
+TARGET_SETS = [[0], [5], [13], [0, 1], [3, 12], [0, 1, 2], [1, 6, 11],
+               [10, 11, 12, 13], [0, 2, 4, 6], [0, 1, 2, 3, 4], [2, 5, 7, 9, 13],
+               [9, 10, 11, 12, 13], [0, 7, 8, 12, 13]]
+
+
+@pytest.mark.parametrize("n", [12, 14])
+@pytest.mark.parametrize("targets", TARGET_SETS, ids=lambda t: "t" + "_".join(map(str, t)))
+def test_mfma_unitary_vs_numpy(qlib, n, targets):
+    if max(targets) >= n:
+        pytest.skip("target beyond n")
+    hip, _ = qlib
+    rng = np.random.default_rng(1000 * n + sum(targets))
+    psi = rand_state(n, rng)
+    U = rand_unitary(len(targets), rng)
+    with qcs_amd.Circuit(n, qcs_amd.PLAIN_FP32) as c:
+        c.set_amplitudes(psi)
+        hip_apply_unitary(hip, c.device_state, targets, U)
+        got = c.amplitudes()
+    ref = apply_dense(psi.astype(np.complex64).astype(np.complex128), n, targets, U)
+    err = np.max(np.abs(got - ref))
+    assert err < 2e-6, err
+
+
+@pytest.mark.parametrize("n,targets", [(5, [0, 1, 2, 3, 4]), (8, [1, 3, 6]),
+                                       (11, [0, 10]), (3, [2])])
+def test_valu_fp32_unitary_small_n(qlib, n, targets):
+    hip, _ = qlib
+    rng = np.random.default_rng(n)
+    psi = rand_state(n, rng)
+    U = rand_unitary(len(targets), rng)
+    with qcs_amd.Circuit(n, qcs_amd.PLAIN_FP32) as c:
+        c.set_amplitudes(psi)
+        hip_apply_unitary(hip, c.device_state, targets, U)
+        got = c.amplitudes()
+    assert np.max(np.abs(got - apply_dense(psi, n, targets, U))) < 2e-6
+
+
+@pytest.mark.parametrize("n,targets", [(6, [0, 1, 2, 3, 4]), (13, [2, 7, 12]),
+                                       (14, [0, 13])])
+def test_fp64_unitary(qlib, n, targets):
+    hip, _ = qlib
+    rng = np.random.default_rng(7 * n)
+    psi = rand_state(n, rng)
+    U = rand_unitary(len(targets), rng)
+    with qcs_amd.Circuit(n) as c:
+        c.set_amplitudes(psi)
+        hip_apply_unitary(hip, c.device_state, targets, U)
+        got = c.amplitudes()
+    np.testing.assert_allclose(got, apply_dense(psi, n, targets, U), rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("n,depth,k", [(12, 3, 5), (13, 4, 4), (16, 5, 5),
+                                       (18, 4, 3), (20, 6, 5), (22, 3, 5)])
+def test_fused_layered_vs_plain_fp64(qlib, n, depth, k):
+    print(f"[fused] layered n={n} depth={depth} k={k}", flush=True)
+    ops = C.layered(n, depth, 2)
+    with qcs_amd.Circuit(n) as ref, \
+            qcs_amd.Circuit(n, qcs_amd.FUSED_FP32 | qcs_amd.FUSE_K(k)) as f:
+        ref.apply(ops)
+        f.apply(ops)
+        omf, mx = f.compare_with(ref)
+        assert omf <= 1e-6, omf
+        assert mx < 1e-5, mx
+        assert 0 < f.get_num_fused_blocks() < len(ops)
+        assert abs(f.norm_sq() - 1.0) < 1e-6
+
+
+@pytest.mark.parametrize("n", [4, 9, 12, 15])
+def test_fused_random_circuits_vs_oracle_and_numpy(qlib, n):
+    kinds = [C.H, C.X, C.Y, C.Z, C.P, C.RX, C.RY, C.RZ, C.CNOT, C.CPHASE, C.SWAP]
+    ops = C.random_circuit(n, 120, 4242 + n, kinds=kinds)
+    ref = HP.simulate(n, ops)
+    with qcs_amd.Circuit(n, qcs_amd.FUSED_FP32) as f:
+        f.apply(ops)
+        got = f.amplitudes()
+    ip = np.vdot(ref, got)
+    assert 1 - abs(ip) ** 2 / (np.vdot(got, got).real) <= 1e-6
+    assert np.max(np.abs(got - ref)) < 1e-5
+
+
+def test_fused_qft_with_swaps_closed_form(qlib):
+    n, b = 14, 9999
+    with qcs_amd.Circuit(n, qcs_amd.FUSED_FP32) as f:
+        for q in range(n):
+            if (b >> q) & 1:
+                f.x(q)
+        f.apply(C.qft(n, swaps=True))
+        got = f.amplitudes()
+    ref = HP.qft_closed_form(n, b, swaps=True)
+    assert np.max(np.abs(got - ref)) < 1e-6
+    assert 1 - abs(np.vdot(ref, got)) ** 2 <= 1e-6
+
+
+def test_fused_observation_flushes_and_measures(qlib, cl):
+    n = 13
+    ops = C.layered(n, 2, 11)
+    with qcs_amd.Circuit(n) as ref, qcs_amd.Circuit(n, qcs_amd.FUSED_FP32) as f:
+        ref.apply(ops)
+        f.apply(ops)
+        assert f.find_most_likely_state() == ref.find_most_likely_state()
+        cl.srand(17)
+        a = [f.measure(q) for q in range(n)]
+        cl.srand(17)
+        b = [ref.measure(q) for q in range(n)]
+        assert a == b
