@@ -1,0 +1,253 @@
+#!/usr/bin/env python3
+"""Amplitude updates/s at 30 qubits: fused FP32 vs plain FP64, at 1-F <= 1e-6.
+
+One step = one full run of the 30-qubit depth-20 layered circuit (H on every
+qubit, then 20 layers of Rz-Ry-Rz on every qubit and a CNOT ladder; angles
+from splitmix64(seed 2); 2,410 gates) on a complex64 state resident in HBM:
+reset to |0...0>, record the gates through the qc_* API, flush (fusion plan,
+FP64 composition of the fused unitaries on the device, one FP32 matrix-core
+pass per fused block, FP64 renormalisation).  value = gates * 2^30 / step
+time.
+
+Beside it, in the same process:
+  * the plain FP64 form of the same circuit (complex128, one kernel per gate),
+    for the speedup and for 1 - F / max-abs of the fused state;
+  * the dominant kernel's roofline: average duration of the fused
+    matrix-core kernel from HIP events recorded on the state's stream inside
+    the timed steps; algorithmic bytes per launch = 2 * 2^30 * 8;
+  * the CPU baseline: the reference's own sequential library (oracle/_ref,
+    built from its sources) on the 20-qubit config-1 circuit, first gates
+    only (a bounded ~15 s sample), 1 core.
+
+Usage: python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+import argparse
+import json
+import math
+import os
+import platform
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "amplitude updates/s at 30 qubits: fused FP32 vs plain FP64, at 1-F<=1e-6"
+HBM_PEAK_GBS = 8000.0
+FP32_MFMA_PEAK_TFLOPS = 157.3
+
+
+def parse():
+    p = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--qubits", type=int, default=30)
+    p.add_argument("--depth", type=int, default=20)
+    p.add_argument("--seed", type=int, default=2)
+    p.add_argument("--fuse-k", type=int, default=5)
+    p.add_argument("--no-plain", action="store_true", help="skip the plain FP64 run")
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--cpu-seconds", type=float, default=15.0)
+    return p.parse_args()
+
+
+def timing(hip, state, cls):
+    import ctypes
+    ms, n, b, f = (ctypes.c_double(), ctypes.c_long(), ctypes.c_double(),
+                   ctypes.c_double())
+    rc = hip.qcs_hip_timing_read(state, cls, ctypes.byref(ms), ctypes.byref(n),
+                                 ctypes.byref(b), ctypes.byref(f))
+    if rc != 0:
+        raise RuntimeError(hip.qcs_hip_last_error())
+    return ms.value, n.value, b.value, f.value
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def cpu_baseline(seconds):
+    """Reference sequential library on the 20-qubit config-1 circuit."""
+    from qcs_amd import circuits as C
+    n = 20
+    ops = C.layered(n, 10, 0)
+    try:
+        from oracle.oracle import RefLib
+        ref = RefLib("libqcsref.so")
+        kind = "reference"
+        what = ("reference sequential build (its sources, -std=c89 -O3 "
+                "-march=x86-64-v3 -mavx2 -mfma)")
+        c = ref.create(n)
+        apply_one = lambda i: ref.apply(c, _slice(ops, i))  # noqa: E731
+        close = lambda: ref.L.qc_destroy(c)  # noqa: E731
+    except (FileNotFoundError, OSError):
+        from oracle.oracle import OracleState
+        st = OracleState(n, compat=True)
+        kind = "port"
+        what = "C restatement of the reference's sequential loop (oracle)"
+        apply_one = lambda i: st.run(_slice(ops, i))  # noqa: E731
+        close = st.close
+    done = 0
+    t0 = time.perf_counter()
+    while done < len(ops) and time.perf_counter() - t0 < seconds:
+        apply_one(done)
+        done += 1
+    dt = time.perf_counter() - t0
+    close()
+    return {
+        "value": done * float(1 << n) / dt,
+        "unit": "amplitude updates/s",
+        "cores": 1,
+        "kind": kind,
+        "sample": (f"first {done} of {len(ops)} gates of the 20-qubit depth-10 "
+                   f"layered circuit (seed 0), {dt:.1f} s, {what}"),
+        "gates_per_s": done / dt,
+        "cpu": cpu_model(),
+        "nproc": os.cpu_count(),
+    }
+
+
+def _slice(ops, i):
+    from qcs_amd import circuits as C
+    o = C.Ops()
+    o.add(ops.op[i], ops.tq[i], ops.cq[i], ops.par[i])
+    return o
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the state-vector path here is single-GPU by design", file=sys.stderr)
+        sys.exit(2)
+    import qcs_amd
+    from qcs_amd import circuits as C
+
+    hip, _ = qcs_amd.load()
+    if not hip.qcs_hip_device_available():
+        raise SystemExit("no HIP device")
+    n, depth = args.qubits, args.depth
+    ops = C.layered(n, depth, args.seed)
+    G, N = len(ops), float(1 << n)
+
+    fused = qcs_amd.Circuit(n, qcs_amd.FUSED_FP32 | qcs_amd.FUSE_K(args.fuse_k))
+
+    def step():
+        fused.set_basis_state(0)
+        fused.apply(ops)
+        fused.flush()
+
+    for _ in range(args.warmup):
+        step()
+    blocks_per_step = fused.get_num_fused_blocks() // max(args.warmup, 1) if args.warmup else None
+    state = fused.device_state
+    hip.qcs_hip_device_sync()
+    hip.qcs_hip_timing_enable(state, 1)
+    b0 = fused.get_num_fused_blocks()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    hip.qcs_hip_device_sync()
+    T = (time.perf_counter() - t0) / args.steps
+    nblocks = (fused.get_num_fused_blocks() - b0) // args.steps
+    f_ms, f_n, f_bytes, f_flops = timing(hip, state, qcs_amd.K_FUSED)
+    c_ms, c_n, _, _ = timing(hip, state, qcs_amd.K_COMPOSE)
+    hip.qcs_hip_timing_enable(state, 0)
+    avg_ms = f_ms / max(f_n, 1)
+    bytes_per_launch = 2.0 * N * 8.0
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    flops_per_launch = f_flops / max(f_n, 1)
+    tflops = flops_per_launch / (avg_ms * 1e-3) / 1e12
+
+    result = {
+        "metric": METRIC,
+        "value": G * N / T,
+        "unit": "amplitude updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": T * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"{n}-qubit layered Rz-Ry-Rz + CNOT-ladder circuit, depth "
+                         f"{depth}, angles splitmix64(seed {args.seed}), complex64 "
+                         f"state ({(8 << n) / 2**30:.0f} GiB), gates fused into "
+                         f"k<={args.fuse_k} unitaries on FP32 MFMA"),
+            "qubits": n,
+            "depth": depth,
+            "gates": G,
+            "fusion_k": args.fuse_k,
+            "fused_blocks": nblocks,
+        },
+        "gates_per_s": G / T,
+        "fused_kernel_ms_per_step": f_ms / args.steps,
+        "compose_ms_per_step": c_ms / args.steps,
+    }
+    result["roofline"] = {
+        "bound": "hbm",
+        "kernel": "k_fused_mfma (complex64 fused k-qubit pass)",
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS,
+        "traffic": None,
+        "launches": f_n,
+        "avg_launch_ms": avg_ms,
+        "mfma_tflops": tflops,
+        "mfma_frac": tflops / FP32_MFMA_PEAK_TFLOPS,
+    }
+    traffic_file = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(traffic_file):
+        with open(traffic_file) as fh:
+            t = json.load(fh).get("k_fused_mfma")
+        if t:
+            result["roofline"]["traffic"] = t.get("bytes_per_launch")
+
+    if not args.no_plain:
+        plain = qcs_amd.Circuit(n)
+        pstate = plain.device_state
+        hip.qcs_hip_timing_enable(pstate, 1)
+        hip.qcs_hip_device_sync()
+        t1 = time.perf_counter()
+        plain.apply(ops)
+        plain.flush()
+        Tp = time.perf_counter() - t1
+        m1, n1, b1, _ = timing(hip, pstate, qcs_amd.K_1Q)
+        m2, n2, b2, _ = timing(hip, pstate, qcs_amd.K_CU)
+        omf, mx = fused.compare_with(plain)
+        result["plain_fp64"] = {
+            "ms": Tp * 1e3,
+            "amp_updates_per_s": G * N / Tp,
+            "gates_per_s": G / Tp,
+            "k_1q_avg_ms": m1 / max(n1, 1),
+            "k_1q_gbs": (b1 / max(n1, 1)) / (m1 / max(n1, 1) * 1e-3) / 1e9 if n1 else None,
+            "k_cu_avg_ms": m2 / max(n2, 1),
+            "k_cu_gbs": (b2 / max(n2, 1)) / (m2 / max(n2, 1) * 1e-3) / 1e9 if n2 else None,
+        }
+        result["plain_fp64"]["k_1q_hbm_frac"] = (result["plain_fp64"]["k_1q_gbs"] or 0) / HBM_PEAK_GBS
+        result["speedup_vs_plain_fp64"] = Tp / T
+        result["one_minus_f"] = omf
+        result["max_abs_err"] = mx
+        plain.close()
+
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    fused.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

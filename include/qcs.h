@@ -116,6 +116,10 @@ void qc_grover_search_iters(t_q_circuit *circuit, long solution_state,
  * qc_run_shots). */
 void qc_sample(t_q_circuit *circuit, int shots, long *indices);
 
+/* Reset the state to the basis state |index> (recorded gates are dropped;
+ * the history is kept). */
+void qc_set_basis_state(t_q_circuit *circuit, long index);
+
 /* Amplitude access: 2*count doubles {re, im} from index `offset`. */
 int qc_get_amplitudes(t_q_circuit *circuit, long offset, long count,
                       double *out);

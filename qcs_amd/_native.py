@@ -127,6 +127,7 @@ _QCS_PROTOS = {
     "qc_compare": (c_double, [_Q, _Q, P(c_double)]),
     "qc_get_num_fused_blocks": (c_long, [_Q]),
     "qc_get_device_state": (c_void_p, [_Q]),
+    "qc_set_basis_state": (None, [_Q, c_long]),
     "qcs_fusion_plan": (c_int, [P(HipGate), c_int, c_int, c_int, P(P(HipBlock)),
                                 P(c_int), P(P(HipGate))]),
 }

@@ -476,6 +476,16 @@ double qc_get_probability(t_q_circuit *circuit, int state) {
   return qc_get_probability_l(circuit, (long)state);
 }
 
+void qc_set_basis_state(t_q_circuit *circuit, long index) {
+  if (!circuit || !circuit->state) return;
+  if (index < 0 || index >= (1L << circuit->num_qubits)) {
+    fprintf(stderr, "Error: Invalid state or basis index\n");
+    return;
+  }
+  circuit->pending_size = 0;
+  check(circuit, qcs_hip_state_set_basis(circuit->state, index), "set basis");
+}
+
 int qc_get_amplitudes(t_q_circuit *circuit, long offset, long count,
                       double *out) {
   if (!circuit || !circuit->state) return QCS_HIP_EINVAL;
