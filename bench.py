@@ -46,7 +46,11 @@ def parse():
     p.add_argument("--qubits", type=int, default=30)
     p.add_argument("--depth", type=int, default=20)
     p.add_argument("--seed", type=int, default=2)
-    p.add_argument("--fuse-k", type=int, default=5)
+    p.add_argument("--fuse-k", type=int, default=0,
+                   help="max qubits per fused block (default: 4 in tile programs, 5 per-block)")
+    p.add_argument("--mode", choices=["tiles", "blocks"], default="blocks",
+                   help="tiles: several fused blocks per HBM pass; blocks: one pass per block")
+    p.add_argument("--tile-threads", type=int, default=1024)
     p.add_argument("--no-plain", action="store_true", help="skip the plain FP64 run")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -139,7 +143,14 @@ def main():
     ops = C.layered(n, depth, args.seed)
     G, N = len(ops), float(1 << n)
 
-    fused = qcs_amd.Circuit(n, qcs_amd.FUSED_FP32 | qcs_amd.FUSE_K(args.fuse_k))
+    opts = qcs_amd.FUSED_FP32
+    if args.fuse_k:
+        opts |= qcs_amd.FUSE_K(args.fuse_k)
+    if args.mode == "blocks":
+        opts |= qcs_amd.FUSE_BLOCKS
+    fused = qcs_amd.Circuit(n, opts)
+    hip.qcs_hip_set_tuning(fused.device_state, qcs_amd.TUNE_TILE_THREADS, args.tile_threads)
+    fuse_k = args.fuse_k or (4 if args.mode == "tiles" else 5)
 
     def step():
         fused.set_basis_state(0)
@@ -148,17 +159,18 @@ def main():
 
     for _ in range(args.warmup):
         step()
-    blocks_per_step = fused.get_num_fused_blocks() // max(args.warmup, 1) if args.warmup else None
     state = fused.device_state
     hip.qcs_hip_device_sync()
     hip.qcs_hip_timing_enable(state, 1)
     b0 = fused.get_num_fused_blocks()
+    p0 = fused.get_num_fused_passes()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
     hip.qcs_hip_device_sync()
     T = (time.perf_counter() - t0) / args.steps
     nblocks = (fused.get_num_fused_blocks() - b0) // args.steps
+    npasses = (fused.get_num_fused_passes() - p0) // args.steps
     f_ms, f_n, f_bytes, f_flops = timing(hip, state, qcs_amd.K_FUSED)
     c_ms, c_n, _, _ = timing(hip, state, qcs_amd.K_COMPOSE)
     hip.qcs_hip_timing_enable(state, 0)
@@ -185,12 +197,17 @@ def main():
             "workload": (f"{n}-qubit layered Rz-Ry-Rz + CNOT-ladder circuit, depth "
                          f"{depth}, angles splitmix64(seed {args.seed}), complex64 "
                          f"state ({(8 << n) / 2**30:.0f} GiB), gates fused into "
-                         f"k<={args.fuse_k} unitaries on FP32 MFMA"),
+                         f"k<={fuse_k} unitaries on FP32 MFMA, "
+                         + ("several per HBM pass (tile programs)" if args.mode == "tiles"
+                            else "one HBM pass each")),
             "qubits": n,
             "depth": depth,
             "gates": G,
-            "fusion_k": args.fuse_k,
+            "fusion_k": fuse_k,
+            "fusion_mode": args.mode,
+            "tile_threads": args.tile_threads,
             "fused_blocks": nblocks,
+            "hbm_passes": npasses,
         },
         "gates_per_s": G / T,
         "fused_kernel_ms_per_step": f_ms / args.steps,
@@ -198,7 +215,8 @@ def main():
     }
     result["roofline"] = {
         "bound": "hbm",
-        "kernel": "k_fused_mfma (complex64 fused k-qubit pass)",
+        "kernel": ("k_tile_program (complex64, fused blocks per tile pass)"
+                   if args.mode == "tiles" else "k_fused_mfma (complex64 fused k-qubit pass)"),
         "achieved": achieved,
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
@@ -212,7 +230,7 @@ def main():
     traffic_file = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(traffic_file):
         with open(traffic_file) as fh:
-            t = json.load(fh).get("k_fused_mfma")
+            t = json.load(fh).get("k_tile_program" if args.mode == "tiles" else "k_fused_mfma")
         if t:
             result["roofline"]["traffic"] = t.get("bytes_per_launch")
 

@@ -94,8 +94,12 @@ void qc_swap(t_q_circuit *circuit, int qubit_a, int qubit_b);
                                     forms only) */
 #define QCS_RENORM_AT_BARRIER 0x8 /* fused form: flush + FP64 renormalise at
                                      every qc_barrier */
+#define QCS_FUSE_BLOCKS 0x10    /* fused form: one HBM pass per fused block
+                                   (default: tile programs, several blocks
+                                   per pass, when num_qubits >= 16) */
 #define QCS_FUSE_K(k) ((k) << 8) /* fused form: max qubits per fused block,
-                                    1..5 (default 5) */
+                                    1..5 (default 4 in tile programs, 5 with
+                                    QCS_FUSE_BLOCKS) */
 t_q_circuit *qc_create_ex(int num_qubits, int options);
 int qc_get_options(t_q_circuit *circuit);
 
@@ -130,8 +134,9 @@ double qc_norm_sq(t_q_circuit *circuit);
 /* 1 - |<a|b>|^2/(|a|^2 |b|^2) of two circuits' states (FP64 on the device);
  * *max_abs (if not NULL) receives the max-abs amplitude difference. */
 double qc_compare(t_q_circuit *a, t_q_circuit *b, double *max_abs);
-/* Number of fused blocks applied so far (fused form). */
+/* Number of fused blocks / HBM passes applied so far (fused form). */
 long qc_get_num_fused_blocks(t_q_circuit *circuit);
+long qc_get_num_fused_passes(t_q_circuit *circuit);
 /* The circuit's device state (a qcs_hip_state *, see qcs_hip.h), after
  * applying every recorded gate; NULL when the circuit has no state. */
 void *qc_get_device_state(t_q_circuit *circuit);

@@ -173,6 +173,40 @@ int qcs_hip_apply_fused(qcs_hip_state *s, const qcs_hip_block *blocks,
 int qcs_hip_apply_unitary(qcs_hip_state *s, int k, const int *targets,
                           const double *u);
 
+/* Tile programs: several fused blocks per HBM pass.  A pass names
+ * `tile_bits` index bits (ascending in `bits`, and containing bits 0..5);
+ * the state is streamed as 2^tile_bits-amplitude tiles and each of the
+ * pass's `num_blocks` consecutive blocks (whose targets lie in `bits`) is
+ * applied to every tile in LDS before the tile is written back.  complex64
+ * states with num_qubits >= tile_bits + 3; tile_bits is 13; blocks of a
+ * pass act on at most QCS_HIP_TILE_KMAX qubits and a pass holds at most
+ * QCS_HIP_TILE_MAX_BLOCKS blocks (their operands stay in LDS).  No reference
+ * counterpart. */
+#define QCS_HIP_TILE_BITS 13
+#define QCS_HIP_TILE_LOW_BITS 6
+#define QCS_HIP_TILE_KMAX 4
+#define QCS_HIP_TILE_MAX_BLOCKS 23
+typedef struct qcs_hip_pass {
+  int tile_bits;
+  int first_block;
+  int num_blocks;
+  int reserved;
+  int bits[16];
+} qcs_hip_pass;
+int qcs_hip_apply_passes(qcs_hip_state *s, const qcs_hip_pass *passes,
+                         int num_passes, const qcs_hip_block *blocks,
+                         int num_blocks, const qcs_hip_gate *gates,
+                         int num_gates);
+
+/* Launch-shape knobs of the state's kernels (for the autotuner).
+ * QCS_HIP_TUNE_TILE_THREADS: 512 or 1024 threads per tile-program
+ * work-group (default 1024); QCS_HIP_TUNE_TILE_GRID: work-groups of the
+ * persistent tile-program grid when one fits per CU (multiple of 8, default
+ * 256; doubled when two fit). */
+#define QCS_HIP_TUNE_TILE_THREADS 0
+#define QCS_HIP_TUNE_TILE_GRID 1
+int qcs_hip_set_tuning(qcs_hip_state *s, int knob, int value);
+
 /* ---- comparison ----------------------------------------------------------
  * FP64 on the device: one_minus_f = 1 - |<a|b>|^2 / (|a|^2 |b|^2) and the
  * max-abs amplitude difference.  States may differ in precision but not in

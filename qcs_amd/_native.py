@@ -45,6 +45,15 @@ class HipBlock(ctypes.Structure):
                 ("reserved", c_int), ("targets", c_int * 8)]
 
 
+class HipPass(ctypes.Structure):
+    _fields_ = [("tile_bits", c_int), ("first_block", c_int), ("num_blocks", c_int),
+                ("reserved", c_int), ("bits", c_int * 16)]
+
+
+TILE_BITS, TILE_LOW_BITS = 13, 6
+TUNE_TILE_THREADS, TUNE_TILE_GRID = 0, 1
+FUSE_BLOCKS = 0x10
+
 _HIP_PROTOS = {
     "qcs_hip_last_error": (ctypes.c_char_p, []),
     "qcs_hip_device_available": (c_int, []),
@@ -75,6 +84,9 @@ _HIP_PROTOS = {
     "qcs_hip_apply_fused": (c_int, [c_void_p, P(HipBlock), c_int, P(HipGate), c_int]),
     "qcs_hip_apply_unitary": (c_int, [c_void_p, c_int, P(c_int), P(c_double)]),
     "qcs_hip_compare": (c_int, [c_void_p, c_void_p, P(c_double), P(c_double)]),
+    "qcs_hip_apply_passes": (c_int, [c_void_p, P(HipPass), c_int, P(HipBlock), c_int,
+                                     P(HipGate), c_int]),
+    "qcs_hip_set_tuning": (c_int, [c_void_p, c_int, c_int]),
     "qcs_hip_timing_enable": (c_int, [c_void_p, c_int]),
     "qcs_hip_timing_read": (c_int, [c_void_p, c_int, P(c_double), P(c_long),
                                     P(c_double), P(c_double)]),
@@ -126,10 +138,14 @@ _QCS_PROTOS = {
     "qc_norm_sq": (c_double, [_Q]),
     "qc_compare": (c_double, [_Q, _Q, P(c_double)]),
     "qc_get_num_fused_blocks": (c_long, [_Q]),
+    "qc_get_num_fused_passes": (c_long, [_Q]),
     "qc_get_device_state": (c_void_p, [_Q]),
     "qc_set_basis_state": (None, [_Q, c_long]),
     "qcs_fusion_plan": (c_int, [P(HipGate), c_int, c_int, c_int, P(P(HipBlock)),
                                 P(c_int), P(P(HipGate))]),
+    "qcs_pass_plan": (c_int, [P(HipGate), c_int, c_int, c_int, c_int, c_int, c_int,
+                              P(P(HipPass)), P(c_int), P(P(HipBlock)), P(c_int),
+                              P(P(HipGate))]),
 }
 
 _hip = None

@@ -116,6 +116,8 @@ static void free_state(qcs_hip_state *s) {
   if (s->chunk_inc) (void)hipFree(s->chunk_inc);
   if (s->u_dev) (void)hipFree(s->u_dev);
   if (s->idx_dev) (void)hipFree(s->idx_dev);
+  if (s->tpass_dev) (void)hipFree(s->tpass_dev);
+  if (s->tblock_dev) (void)hipFree(s->tblock_dev);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
@@ -162,6 +164,40 @@ static int copy_out(qcs_hip_state *s, const void *dev, long off, long count,
       for (long i = 0; i < 2 * m; ++i) out[2 * done + i] = double(f[i]);
     }
     done += m;
+  }
+  return 0;
+}
+
+static int validate_blocks(const qcs_hip_state *s, const qcs_hip_block *blocks,
+                           int num_blocks, const qcs_hip_gate *gates,
+                           int num_gates) {
+  if (num_blocks < 0 || num_gates < 0 || (num_blocks > 0 && !blocks) ||
+      (num_gates > 0 && !gates))
+    return fail(QCS_HIP_EINVAL, "bad block/gate arrays");
+  for (int b = 0; b < num_blocks; ++b) {
+    const qcs_hip_block &bl = blocks[b];
+    if (bl.k < 1 || bl.k > QCS_HIP_MAX_FUSE || bl.k > s->n)
+      return fail(QCS_HIP_EINVAL, "block %d: k=%d outside [1,%d]", b, bl.k,
+                  QCS_HIP_MAX_FUSE);
+    for (int q = 0; q < bl.k; ++q) {
+      if (bl.targets[q] < 0 || bl.targets[q] >= s->n ||
+          (q > 0 && bl.targets[q] <= bl.targets[q - 1]))
+        return fail(QCS_HIP_EINVAL, "block %d: targets not ascending in range", b);
+    }
+    if (bl.first_gate < 0 || bl.num_gates < 0 ||
+        bl.first_gate + bl.num_gates > num_gates)
+      return fail(QCS_HIP_EINVAL, "block %d: gate range outside the gate array", b);
+    for (int g = bl.first_gate; g < bl.first_gate + bl.num_gates; ++g) {
+      const qcs_hip_gate &gt = gates[g];
+      bool t_in = false, c_in = gt.kind == QCS_HIP_G_1Q;
+      for (int q = 0; q < bl.k; ++q) {
+        if (bl.targets[q] == gt.target) t_in = true;
+        if (bl.targets[q] == gt.control) c_in = true;
+      }
+      if (gt.kind < QCS_HIP_G_1Q || gt.kind > QCS_HIP_G_SWAP || !t_in || !c_in ||
+          (gt.kind != QCS_HIP_G_1Q && gt.control == gt.target))
+        return fail(QCS_HIP_EINVAL, "block %d: gate %d acts outside the block", b, g);
+    }
   }
   return 0;
 }
@@ -430,34 +466,7 @@ int qcs_hip_apply_fused(qcs_hip_state *s, const qcs_hip_block *blocks,
   if ((rc = check_state(s))) return rc;
   if (s->flags & QCS_HIP_COMPAT)
     return fail(QCS_HIP_EUNSUP, "fusion needs textbook semantics");
-  if (num_blocks < 0 || num_gates < 0 || (num_blocks > 0 && !blocks) ||
-      (num_gates > 0 && !gates))
-    return fail(QCS_HIP_EINVAL, "bad block/gate arrays");
-  for (int b = 0; b < num_blocks; ++b) {
-    const qcs_hip_block &bl = blocks[b];
-    if (bl.k < 1 || bl.k > QCS_HIP_MAX_FUSE || bl.k > s->n)
-      return fail(QCS_HIP_EINVAL, "block %d: k=%d outside [1,%d]", b, bl.k,
-                  QCS_HIP_MAX_FUSE);
-    for (int q = 0; q < bl.k; ++q) {
-      if (bl.targets[q] < 0 || bl.targets[q] >= s->n ||
-          (q > 0 && bl.targets[q] <= bl.targets[q - 1]))
-        return fail(QCS_HIP_EINVAL, "block %d: targets not ascending in range", b);
-    }
-    if (bl.first_gate < 0 || bl.num_gates < 0 ||
-        bl.first_gate + bl.num_gates > num_gates)
-      return fail(QCS_HIP_EINVAL, "block %d: gate range outside the gate array", b);
-    for (int g = bl.first_gate; g < bl.first_gate + bl.num_gates; ++g) {
-      const qcs_hip_gate &gt = gates[g];
-      bool t_in = false, c_in = gt.kind == QCS_HIP_G_1Q;
-      for (int q = 0; q < bl.k; ++q) {
-        if (bl.targets[q] == gt.target) t_in = true;
-        if (bl.targets[q] == gt.control) c_in = true;
-      }
-      if (gt.kind < QCS_HIP_G_1Q || gt.kind > QCS_HIP_G_SWAP || !t_in || !c_in ||
-          (gt.kind != QCS_HIP_G_1Q && gt.control == gt.target))
-        return fail(QCS_HIP_EINVAL, "block %d: gate %d acts outside the block", b, g);
-    }
-  }
+  if ((rc = validate_blocks(s, blocks, num_blocks, gates, num_gates))) return rc;
   return fused_apply(s, blocks, num_blocks, gates, num_gates);
 }
 
@@ -473,6 +482,74 @@ int qcs_hip_apply_unitary(qcs_hip_state *s, int k, const int *targets,
     if (targets[q] < 0 || targets[q] >= s->n || (q > 0 && targets[q] <= targets[q - 1]))
       return fail(QCS_HIP_EINVAL, "apply_unitary: targets not ascending in range");
   return fused_apply_unitary(s, k, targets, u);
+}
+
+int qcs_hip_apply_passes(qcs_hip_state *s, const qcs_hip_pass *passes,
+                         int num_passes, const qcs_hip_block *blocks,
+                         int num_blocks, const qcs_hip_gate *gates,
+                         int num_gates) {
+  int rc;
+  if ((rc = check_state(s))) return rc;
+  if (s->flags & QCS_HIP_COMPAT)
+    return fail(QCS_HIP_EUNSUP, "fusion needs textbook semantics");
+  if (s->precision != QCS_HIP_C64)
+    return fail(QCS_HIP_EUNSUP, "tile programs need a complex64 state");
+  if (num_passes < 0 || (num_passes > 0 && !passes))
+    return fail(QCS_HIP_EINVAL, "bad pass array");
+  // blocks and gates are checked as for qcs_hip_apply_fused
+  int covered = 0;
+  for (int p = 0; p < num_passes; ++p) {
+    const qcs_hip_pass &ps = passes[p];
+    if (ps.tile_bits != QCS_HIP_TILE_BITS || s->n < ps.tile_bits + 3)
+      return fail(QCS_HIP_EINVAL, "pass %d: tile_bits %d (need %d and n >= %d)",
+                  p, ps.tile_bits, QCS_HIP_TILE_BITS, QCS_HIP_TILE_BITS + 3);
+    for (int j = 0; j < ps.tile_bits; ++j) {
+      if (ps.bits[j] < 0 || ps.bits[j] >= s->n || (j > 0 && ps.bits[j] <= ps.bits[j - 1]))
+        return fail(QCS_HIP_EINVAL, "pass %d: bits not ascending in range", p);
+      if (j < QCS_HIP_TILE_LOW_BITS && ps.bits[j] != j)
+        return fail(QCS_HIP_EINVAL, "pass %d: bits must start with 0..%d", p,
+                    QCS_HIP_TILE_LOW_BITS - 1);
+    }
+    if (ps.first_block != covered || ps.num_blocks < 0 ||
+        ps.first_block + ps.num_blocks > num_blocks)
+      return fail(QCS_HIP_EINVAL, "pass %d: blocks must follow the previous pass", p);
+    if (ps.num_blocks > tile_max_blocks())
+      return fail(QCS_HIP_EINVAL, "pass %d: %d blocks (at most %d)", p,
+                  ps.num_blocks, tile_max_blocks());
+    for (int b = ps.first_block; b < ps.first_block + ps.num_blocks; ++b) {
+      if (blocks[b].k > QCS_HIP_TILE_KMAX)
+        return fail(QCS_HIP_EINVAL, "pass %d: block %d has k=%d (at most %d)", p,
+                    b, blocks[b].k, QCS_HIP_TILE_KMAX);
+      for (int q = 0; q < blocks[b].k && q < QCS_HIP_MAX_FUSE; ++q) {
+        bool in = false;
+        for (int j = 0; j < ps.tile_bits; ++j) in = in || ps.bits[j] == blocks[b].targets[q];
+        if (!in) return fail(QCS_HIP_EINVAL, "pass %d: block %d targets a bit outside the tile", p, b);
+      }
+    }
+    covered += ps.num_blocks;
+  }
+  if (covered != num_blocks)
+    return fail(QCS_HIP_EINVAL, "passes cover %d of %d blocks", covered, num_blocks);
+  if ((rc = validate_blocks(s, blocks, num_blocks, gates, num_gates))) return rc;
+  return tiles_apply(s, passes, num_passes, blocks, num_blocks, gates, num_gates);
+}
+
+int qcs_hip_set_tuning(qcs_hip_state *s, int knob, int value) {
+  int rc;
+  if ((rc = check_state(s))) return rc;
+  if (knob == QCS_HIP_TUNE_TILE_THREADS) {
+    if (value != 512 && value != 1024)
+      return fail(QCS_HIP_EINVAL, "tile threads must be 512 or 1024");
+    s->tile_threads = value;
+    return 0;
+  }
+  if (knob == QCS_HIP_TUNE_TILE_GRID) {
+    if (value < 8 || value > 65536 || (value & 7))
+      return fail(QCS_HIP_EINVAL, "tile grid must be a multiple of 8 in [8, 65536]");
+    s->tile_grid = value;
+    return 0;
+  }
+  return fail(QCS_HIP_EINVAL, "unknown tuning knob %d", knob);
 }
 
 int qcs_hip_compare(qcs_hip_state *a, qcs_hip_state *b, double *one_minus_f,

@@ -376,9 +376,12 @@ static int ensure_wbuf(qcs_hip_state *s, size_t nb) {
   return 0;
 }
 
-int fused_apply(qcs_hip_state *s, const qcs_hip_block *blocks, int nb,
-                const qcs_hip_gate *gates, int ng) {
-  if (nb <= 0) return 0;
+// Upload the gate list and blocks, compose every block's unitary on the
+// device (W into wbuf slot b, U into ubuf slot b).  `extra` bytes of pinned
+// staging after the gates and blocks are left for the caller's own uploads.
+int compose_blocks(qcs_hip_state *s, const qcs_hip_block *blocks, int nb,
+                   const qcs_hip_gate *gates, int ng, size_t extra,
+                   char **extra_stage) {
   // the previous flush may still read the device gate list
   QCS_HIP_TRY(hipStreamSynchronize(s->stream));
   const size_t gbytes = sizeof(qcs_hip_gate) * size_t(ng > 0 ? ng : 1);
@@ -387,8 +390,9 @@ int fused_apply(qcs_hip_state *s, const qcs_hip_block *blocks, int nb,
   if ((rc = grow(&s->gates_dev, &s->gates_cap, gbytes))) return rc;
   if ((rc = grow(&s->blocks_dev, &s->blocks_cap, bbytes))) return rc;
   if ((rc = ensure_wbuf(s, size_t(nb)))) return rc;
-  if ((rc = ensure_pinned(s, gbytes + bbytes))) return rc;
+  if ((rc = ensure_pinned(s, gbytes + bbytes + extra))) return rc;
   char *stage = reinterpret_cast<char *>(s->host);
+  if (extra_stage) *extra_stage = stage + gbytes + bbytes;
   if (ng > 0) std::memcpy(stage, gates, gbytes);
   std::memcpy(stage + gbytes, blocks, bbytes);
   if (ng > 0)
@@ -403,8 +407,14 @@ int fused_apply(qcs_hip_state *s, const qcs_hip_block *blocks, int nb,
                      static_cast<const qcs_hip_block *>(s->blocks_dev), s->wbuf,
                      s->ubuf);
   if ((rc = launch_check("k_compose"))) return rc;
-  if ((rc = timing_end(s, &tl))) return rc;
+  return timing_end(s, &tl);
+}
 
+int fused_apply(qcs_hip_state *s, const qcs_hip_block *blocks, int nb,
+                const qcs_hip_gate *gates, int ng) {
+  if (nb <= 0) return 0;
+  int rc = compose_blocks(s, blocks, nb, gates, ng, 0, nullptr);
+  if (rc) return rc;
   for (int b = 0; b < nb; ++b) {
     rc = apply_block(s, blocks[b].k, blocks[b].targets,
                      s->wbuf + size_t(b) * kWFloats,

@@ -61,6 +61,13 @@ struct qcs_hip_state {
   double *u_dev = nullptr;
   long *idx_dev = nullptr;
   size_t shots_cap = 0;
+  // tile-program buffers and tuning
+  void *tpass_dev = nullptr;
+  size_t tpass_cap = 0;
+  void *tblock_dev = nullptr;
+  size_t tblock_cap = 0;
+  int tile_threads = 1024;  // work-group size of the tile-program kernel
+  int tile_grid = 256;      // persistent grid of the tile-program kernel
   // kernel timing
   int timing = 0;
   std::vector<qcs::TimedLaunch> pending_events;
@@ -117,6 +124,15 @@ int fused_apply(qcs_hip_state *s, const qcs_hip_block *blocks, int nb,
                 const qcs_hip_gate *gates, int ng);
 int fused_apply_unitary(qcs_hip_state *s, int k, const int *targets,
                         const double *u);
+int compose_blocks(qcs_hip_state *s, const qcs_hip_block *blocks, int nb,
+                   const qcs_hip_gate *gates, int ng, size_t extra,
+                   char **extra_stage);
+
+// kernels in tiles.hip
+int tile_max_blocks();
+int tiles_apply(qcs_hip_state *s, const qcs_hip_pass *passes, int np,
+                const qcs_hip_block *blocks, int nb, const qcs_hip_gate *gates,
+                int ng);
 
 __host__ __device__ inline uint64_t ceil_div(uint64_t a, uint64_t b) {
   return (a + b - 1) / b;

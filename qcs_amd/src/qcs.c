@@ -135,9 +135,18 @@ static void record(t_q_circuit *c, int kind, int target, int control,
   if (m) memcpy(g->m, m, sizeof(g->m));
 }
 
+/* tile programs (several blocks per HBM pass) apply to complex64 fused
+ * states large enough for the tile kernel, unless QCS_FUSE_BLOCKS asks for
+ * one pass per block */
+static int use_tiles(const t_q_circuit *c) {
+  return is_fused(c) && !(c->options & QCS_FUSE_BLOCKS) &&
+         c->num_qubits >= QCS_HIP_TILE_BITS + 3;
+}
+
 static int fuse_k(const t_q_circuit *c) {
   int k = (c->options >> 8) & 0xff;
-  if (k < 1 || k > QCS_HIP_MAX_FUSE) k = QCS_HIP_MAX_FUSE;
+  if (k < 1 || k > QCS_HIP_MAX_FUSE) k = use_tiles(c) ? QCS_HIP_TILE_KMAX : QCS_HIP_MAX_FUSE;
+  if (use_tiles(c) && k > QCS_HIP_TILE_KMAX) k = QCS_HIP_TILE_KMAX;
   if (k > c->num_qubits) k = c->num_qubits;
   return k;
 }
@@ -145,21 +154,37 @@ static int fuse_k(const t_q_circuit *c) {
 static void flush_pending(t_q_circuit *c) {
   qcs_hip_block *blocks = NULL;
   qcs_hip_gate *ordered = NULL;
-  int nb = 0;
+  qcs_hip_pass *passes = NULL;
+  int nb = 0, np = 0, rc;
   if (!c->state || c->pending_size == 0) return;
-  if (qcs_fusion_plan(c->pending, c->pending_size, c->num_qubits, fuse_k(c),
-                      &blocks, &nb, &ordered) != 0) {
+  if (use_tiles(c))
+    rc = qcs_pass_plan(c->pending, c->pending_size, c->num_qubits,
+                       QCS_HIP_TILE_BITS, QCS_HIP_TILE_LOW_BITS, fuse_k(c),
+                       QCS_HIP_TILE_MAX_BLOCKS, &passes, &np, &blocks, &nb,
+                       &ordered);
+  else
+    rc = qcs_fusion_plan(c->pending, c->pending_size, c->num_qubits,
+                         fuse_k(c), &blocks, &nb, &ordered);
+  if (rc != 0) {
     fprintf(stderr, "Error: fusion planning failed.\n");
     c->error = QCS_HIP_ENOMEM;
   } else {
-    if (check(c, qcs_hip_apply_fused(c->state, blocks, nb, ordered,
-                                     c->pending_size),
-              "fused apply") == 0) {
+    if (passes)
+      rc = check(c, qcs_hip_apply_passes(c->state, passes, np, blocks, nb,
+                                         ordered, c->pending_size),
+                 "tile programs");
+    else
+      rc = check(c, qcs_hip_apply_fused(c->state, blocks, nb, ordered,
+                                        c->pending_size),
+                 "fused apply");
+    if (rc == 0) {
       c->fused_blocks += nb;
+      c->fused_passes += passes ? np : nb;
       /* FP64 norm reduction + renormalisation, decided on the device */
       check(c, qcs_hip_normalize(c->state, NULL), "renormalise");
     }
   }
+  free(passes);
   free(blocks);
   free(ordered);
   c->pending_size = 0;
@@ -522,6 +547,10 @@ double qc_compare(t_q_circuit *a, t_q_circuit *b, double *max_abs) {
 
 long qc_get_num_fused_blocks(t_q_circuit *circuit) {
   return circuit ? circuit->fused_blocks : 0;
+}
+
+long qc_get_num_fused_passes(t_q_circuit *circuit) {
+  return circuit ? circuit->fused_passes : 0;
 }
 
 void *qc_get_device_state(t_q_circuit *circuit) {

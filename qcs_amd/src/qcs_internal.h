@@ -23,6 +23,7 @@ struct t_q_circuit {
   int pending_size;
   int pending_capacity;
   long fused_blocks;
+  long fused_passes;
 };
 
 /* Greedy fusion planner (fusion.c).
@@ -34,5 +35,14 @@ struct t_q_circuit {
 int qcs_fusion_plan(const qcs_hip_gate *gates, int ng, int n, int kmax,
                     qcs_hip_block **blocks, int *nblocks,
                     qcs_hip_gate **ordered);
+
+/* Pass planner for tile programs (fusion.c): passes of `tile_bits` qubits
+ * that always include qubits 0..low_bits-1, each holding blocks of at most
+ * kmax qubits; gates re-ordered so every block's gates are contiguous and
+ * blocks of a pass consecutive. */
+int qcs_pass_plan(const qcs_hip_gate *gates, int ng, int n, int tile_bits,
+                  int low_bits, int kmax, int max_blocks, qcs_hip_pass **passes,
+                  int *npasses, qcs_hip_block **blocks, int *nblocks,
+                  qcs_hip_gate **ordered);
 
 #endif

@@ -103,6 +103,52 @@ def test_plan_state_level_12q():
     np.testing.assert_allclose(psi, HP.simulate(n, ops), atol=1e-12)
 
 
+def pass_plan(ops, n, tb=13, low=6, k=4, max_blocks=23):
+    _, qcs = qcs_amd.load()
+    gates = to_gates(ops)
+    pp = ctypes.POINTER(qcs_amd.HipPass)()
+    pb = ctypes.POINTER(HipBlock)()
+    pg = ctypes.POINTER(HipGate)()
+    np_, nb = ctypes.c_int(), ctypes.c_int()
+    rc = qcs.qcs_pass_plan(gates, len(ops), n, tb, low, k, max_blocks, ctypes.byref(pp),
+                           ctypes.byref(np_), ctypes.byref(pb), ctypes.byref(nb),
+                           ctypes.byref(pg))
+    assert rc == 0
+    return ([pp[i] for i in range(np_.value)], [pb[i] for i in range(nb.value)],
+            [pg[i] for i in range(len(ops))])
+
+
+@pytest.mark.parametrize("n,tb,low,k,mb", [(7, 5, 2, 2, 23), (8, 6, 3, 3, 2),
+                                          (9, 7, 3, 4, 1), (9, 7, 3, 4, 23)])
+def test_pass_plan_reproduces_circuit(n, tb, low, k, mb):
+    kinds = [C.H, C.X, C.Y, C.Z, C.P, C.RX, C.RY, C.RZ, C.CNOT, C.CPHASE, C.SWAP]
+    for seed in range(3):
+        ops = C.random_circuit(n, 60, 99 + 13 * n + seed, kinds=kinds)
+        passes, blocks, ordered = pass_plan(ops, n, tb, low, k, mb)
+        assert max(p.num_blocks for p in passes) <= mb
+        U = np.eye(1 << n, dtype=complex)
+        nextb = 0
+        for ps in passes:
+            bits = [ps.bits[j] for j in range(ps.tile_bits)]
+            assert ps.tile_bits == tb and bits == sorted(set(bits))
+            assert bits[:low] == list(range(low))
+            assert ps.first_block == nextb
+            nextb += ps.num_blocks
+            for b in blocks[ps.first_block:ps.first_block + ps.num_blocks]:
+                tg = [b.targets[i] for i in range(b.k)]
+                assert 1 <= b.k <= k and set(tg) <= set(bits)
+                for g in ordered[b.first_gate:b.first_gate + b.num_gates]:
+                    U = gate_matrix_full(n, g) @ U
+        assert nextb == len(blocks)
+        assert sum(b.num_gates for b in blocks) == len(ops)
+        np.testing.assert_allclose(U, HP.dense_unitary(n, ops), atol=1e-12)
+
+
+def test_pass_plan_layered_30q_counts():
+    passes, blocks, _ = pass_plan(C.layered(30, 20, 2), 30)
+    assert (len(passes), len(blocks)) == (38, 189)
+
+
 def test_circuit_gate_counts():
     assert C.gate_count_layered(20, 10) == len(C.layered(20, 10, 0)) == 810
     assert C.gate_count_layered(28, 20) == len(C.layered(28, 20, 0)) == 2248

@@ -134,6 +134,71 @@ def test_fused_random_circuits_vs_oracle_and_numpy(qlib, n):
     assert np.max(np.abs(got - ref)) < 1e-5
 
 
+@pytest.mark.parametrize("n,threads", [(16, 512), (17, 1024), (20, 512), (20, 1024)])
+def test_tile_programs_equal_single_block_passes(qlib, n, threads):
+    """Tile programs apply the same composed blocks, in the same order, with
+    the same MFMA accumulation as one-pass-per-block: bit-identical."""
+    from test_fusion_plan import pass_plan, to_gates  # noqa: F401
+    hip, _ = qlib
+    print(f"[fused] tile programs n={n} threads={threads}", flush=True)
+    kinds = [C.H, C.X, C.Y, C.Z, C.P, C.RX, C.RY, C.RZ, C.CNOT, C.CPHASE, C.SWAP]
+    ops = C.random_circuit(n, 300, 31337 + n, kinds=kinds)
+    passes, blocks, ordered = pass_plan(ops, n)
+    P = (qcs_amd.HipPass * len(passes))(*passes)
+    B = (qcs_amd.HipBlock * len(blocks))(*blocks)
+    G = (qcs_amd.HipGate * len(ordered))(*ordered)
+    psi = rand_state(n, np.random.default_rng(n))
+    with qcs_amd.Circuit(n, qcs_amd.PLAIN_FP32) as a, \
+            qcs_amd.Circuit(n, qcs_amd.PLAIN_FP32) as b:
+        a.set_amplitudes(psi)
+        b.set_amplitudes(psi)
+        assert hip.qcs_hip_set_tuning(a.device_state, qcs_amd.TUNE_TILE_THREADS, threads) == 0
+        rc = hip.qcs_hip_apply_passes(a.device_state, P, len(passes), B, len(blocks),
+                                      G, len(ordered))
+        assert rc == 0, hip.qcs_hip_last_error()
+        rc = hip.qcs_hip_apply_fused(b.device_state, B, len(blocks), G, len(ordered))
+        assert rc == 0, hip.qcs_hip_last_error()
+        got, want = a.amplitudes(), b.amplitudes()
+    assert np.array_equal(got, want)
+    ref = HP.simulate(n, ops, psi.astype(np.complex64).astype(complex))
+    assert 1 - abs(np.vdot(ref, got)) ** 2 / np.vdot(got, got).real <= 1e-6
+
+
+def test_apply_passes_rejects_bad_plans(qlib):
+    from test_fusion_plan import pass_plan
+    hip, _ = qlib
+    n = 16
+    ops = C.layered(n, 2, 5)
+    passes, blocks, ordered = pass_plan(ops, n)
+    B = (qcs_amd.HipBlock * len(blocks))(*blocks)
+    G = (qcs_amd.HipGate * len(ordered))(*ordered)
+    with qcs_amd.Circuit(n, qcs_amd.PLAIN_FP32) as a:
+        bad = (qcs_amd.HipPass * len(passes))(*passes)
+        bad[0].bits[0] = 1  # tile must start with bits 0..5
+        assert hip.qcs_hip_apply_passes(a.device_state, bad, len(passes), B,
+                                        len(blocks), G, len(ordered)) == -1
+        bad = (qcs_amd.HipPass * len(passes))(*passes)
+        bad[0].num_blocks += 1  # blocks not covered exactly once
+        assert hip.qcs_hip_apply_passes(a.device_state, bad, len(passes), B,
+                                        len(blocks), G, len(ordered)) == -1
+    with qcs_amd.Circuit(n) as c128:
+        good = (qcs_amd.HipPass * len(passes))(*passes)
+        assert hip.qcs_hip_apply_passes(c128.device_state, good, len(passes), B,
+                                        len(blocks), G, len(ordered)) == -5
+
+
+@pytest.mark.parametrize("n,depth", [(16, 6), (20, 8), (24, 4)])
+def test_fused_tile_mode_layered_vs_plain(qlib, n, depth):
+    print(f"[fused] tile mode layered n={n}", flush=True)
+    ops = C.layered(n, depth, 3)
+    with qcs_amd.Circuit(n) as ref, qcs_amd.Circuit(n, qcs_amd.FUSED_FP32) as f:
+        ref.apply(ops)
+        f.apply(ops)
+        omf, mx = f.compare_with(ref)
+        assert omf <= 1e-6 and mx < 1e-5, (omf, mx)
+        assert f.get_num_fused_passes() < f.get_num_fused_blocks()
+
+
 def test_fused_qft_with_swaps_closed_form(qlib):
     n, b = 14, 9999
     with qcs_amd.Circuit(n, qcs_amd.FUSED_FP32) as f:
