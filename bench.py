@@ -146,8 +146,8 @@ def main():
     opts = qcs_amd.FUSED_FP32
     if args.fuse_k:
         opts |= qcs_amd.FUSE_K(args.fuse_k)
-    if args.mode == "blocks":
-        opts |= qcs_amd.FUSE_BLOCKS
+    if args.mode == "tiles":
+        opts |= qcs_amd.FUSE_TILES
     fused = qcs_amd.Circuit(n, opts)
     hip.qcs_hip_set_tuning(fused.device_state, qcs_amd.TUNE_TILE_THREADS, args.tile_threads)
     fuse_k = args.fuse_k or (4 if args.mode == "tiles" else 5)

@@ -94,12 +94,12 @@ void qc_swap(t_q_circuit *circuit, int qubit_a, int qubit_b);
                                     forms only) */
 #define QCS_RENORM_AT_BARRIER 0x8 /* fused form: flush + FP64 renormalise at
                                      every qc_barrier */
-#define QCS_FUSE_BLOCKS 0x10    /* fused form: one HBM pass per fused block
-                                   (default: tile programs, several blocks
-                                   per pass, when num_qubits >= 16) */
+#define QCS_FUSE_TILES 0x10     /* fused form: tile programs -- several
+                                   fused blocks per HBM pass (num_qubits >=
+                                   16); default: one HBM pass per block */
 #define QCS_FUSE_K(k) ((k) << 8) /* fused form: max qubits per fused block,
-                                    1..5 (default 4 in tile programs, 5 with
-                                    QCS_FUSE_BLOCKS) */
+                                    1..5 (default 5; at most 4 with
+                                    QCS_FUSE_TILES) */
 t_q_circuit *qc_create_ex(int num_qubits, int options);
 int qc_get_options(t_q_circuit *circuit);
 

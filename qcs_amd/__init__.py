@@ -10,7 +10,7 @@ import ctypes
 import numpy as np
 
 from . import circuits
-from ._native import (C64, C128, FUSE_BLOCKS, FUSED_FP32, FUSE_K, HIP_COMPAT,
+from ._native import (C64, C128, FUSE_TILES, FUSED_FP32, FUSE_K, HIP_COMPAT,
                       K_1Q, K_COMPOSE, K_COUNT, K_CU, K_FUSED, K_OTHER,
                       K_REDUCE, PLAIN_FP32, PLAIN_FP64, REFERENCE_COMPAT,
                       RENORM_AT_BARRIER, TUNE_TILE_GRID, TUNE_TILE_THREADS,
@@ -18,7 +18,7 @@ from ._native import (C64, C128, FUSE_BLOCKS, FUSED_FP32, FUSE_K, HIP_COMPAT,
 
 __all__ = ["Circuit", "load", "libc", "circuits", "PLAIN_FP64", "PLAIN_FP32",
            "FUSED_FP32", "REFERENCE_COMPAT", "RENORM_AT_BARRIER", "FUSE_K",
-           "FUSE_BLOCKS", "HipPass", "TUNE_TILE_THREADS", "TUNE_TILE_GRID",
+           "FUSE_TILES", "HipPass", "TUNE_TILE_THREADS", "TUNE_TILE_GRID",
            "C128", "C64", "HIP_COMPAT", "HipGate", "HipBlock", "K_1Q", "K_CU",
            "K_FUSED", "K_COMPOSE", "K_REDUCE", "K_OTHER", "K_COUNT"]
 

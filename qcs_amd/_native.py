@@ -52,7 +52,7 @@ class HipPass(ctypes.Structure):
 
 TILE_BITS, TILE_LOW_BITS = 13, 6
 TUNE_TILE_THREADS, TUNE_TILE_GRID = 0, 1
-FUSE_BLOCKS = 0x10
+FUSE_TILES = 0x10
 
 _HIP_PROTOS = {
     "qcs_hip_last_error": (ctypes.c_char_p, []),

@@ -136,10 +136,10 @@ static void record(t_q_circuit *c, int kind, int target, int control,
 }
 
 /* tile programs (several blocks per HBM pass) apply to complex64 fused
- * states large enough for the tile kernel, unless QCS_FUSE_BLOCKS asks for
- * one pass per block */
+ * states large enough for the tile kernel when QCS_FUSE_TILES asks for
+ * them */
 static int use_tiles(const t_q_circuit *c) {
-  return is_fused(c) && !(c->options & QCS_FUSE_BLOCKS) &&
+  return is_fused(c) && (c->options & QCS_FUSE_TILES) &&
          c->num_qubits >= QCS_HIP_TILE_BITS + 3;
 }
 

@@ -191,7 +191,8 @@ def test_apply_passes_rejects_bad_plans(qlib):
 def test_fused_tile_mode_layered_vs_plain(qlib, n, depth):
     print(f"[fused] tile mode layered n={n}", flush=True)
     ops = C.layered(n, depth, 3)
-    with qcs_amd.Circuit(n) as ref, qcs_amd.Circuit(n, qcs_amd.FUSED_FP32) as f:
+    with qcs_amd.Circuit(n) as ref, \
+            qcs_amd.Circuit(n, qcs_amd.FUSED_FP32 | qcs_amd.FUSE_TILES) as f:
         ref.apply(ops)
         f.apply(ops)
         omf, mx = f.compare_with(ref)
