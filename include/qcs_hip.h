@@ -205,6 +205,14 @@ int qcs_hip_apply_passes(qcs_hip_state *s, const qcs_hip_pass *passes,
  * 256; doubled when two fit). */
 #define QCS_HIP_TUNE_TILE_THREADS 0
 #define QCS_HIP_TUNE_TILE_GRID 1
+/* QCS_HIP_TUNE_BLOCK_VARIANT(k): launch variant of the one-block-per-pass
+ * matrix-core kernel for blocks of k qubits (1..5), one of: */
+#define QCS_HIP_TUNE_BLOCK_VARIANT(k) (8 + (k))
+#define QCS_HIP_BLOCK_TB12_256 0      /* 4096-amplitude tiles, 256 threads */
+#define QCS_HIP_BLOCK_TB12_256_PIPE 1 /* same, next tile's loads pipelined */
+#define QCS_HIP_BLOCK_TB13_256_PIPE 2 /* 8192-amplitude tiles, 256 threads */
+#define QCS_HIP_BLOCK_TB13_512_PIPE 3 /* 8192-amplitude tiles, 512 threads
+                                         (default) */
 int qcs_hip_set_tuning(qcs_hip_state *s, int knob, int value);
 
 /* ---- comparison ----------------------------------------------------------

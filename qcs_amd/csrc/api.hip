@@ -543,6 +543,13 @@ int qcs_hip_set_tuning(qcs_hip_state *s, int knob, int value) {
     s->tile_threads = value;
     return 0;
   }
+  if (knob >= QCS_HIP_TUNE_BLOCK_VARIANT(1) &&
+      knob <= QCS_HIP_TUNE_BLOCK_VARIANT(QCS_HIP_MAX_FUSE)) {
+    if (value < QCS_HIP_BLOCK_TB12_256 || value > QCS_HIP_BLOCK_TB13_512_PIPE)
+      return fail(QCS_HIP_EINVAL, "unknown block variant %d", value);
+    s->block_variant[knob - QCS_HIP_TUNE_BLOCK_VARIANT(0)] = value;
+    return 0;
+  }
   if (knob == QCS_HIP_TUNE_TILE_GRID) {
     if (value < 8 || value > 65536 || (value & 7))
       return fail(QCS_HIP_EINVAL, "tile grid must be a multiple of 8 in [8, 65536]");

@@ -8,17 +8,10 @@
 //        {re, im} rows/columns: W[2i][2j] = Ur, W[2i][2j+1] = -Ui,
 //        W[2i+1][2j] = Ui, W[2i+1][2j+1] = Ur  (the four real products),
 //      - U itself in FP64 for the VALU kernel.
-// 2. k_fused_mfma applies W to a complex64 state as a GEMM on the FP32 matrix
-//    cores.  A work-group owns a tile of 4096 amplitudes: the 2^k rows are the
-//    target-bit patterns, the 2^(12-k) columns the lowest 12-k non-target
-//    bits.  The tile's 12 index bits S = T + those columns always include bit
-//    0, so each thread moves 16-byte runs of two amplitudes and every wave
-//    touches whole cache lines.  Global -> LDS in address order, MFMA
-//    32x32x2 f32 with W held in registers as the A operand and the LDS tile
-//    as B, accumulators back to LDS, LDS -> global in address order.  With
-//    interleaved rows a complex64 amplitude is two consecutive real rows,
-//    so the B operand of K-step s is {Re, Im} of complex row s and each
-//    32-row accumulator block holds whole complex outputs.
+// 2. Each block is then applied in one HBM pass: complex64 states with
+//    n >= 12 by the matrix-core kernel of blockpass.hip (k_fused_mfma),
+//    other states by k_fused_valu below.  Tile programs (tiles.hip) apply
+//    several composed blocks per pass instead.
 #include <cstring>
 
 #include "device.h"
@@ -26,16 +19,8 @@
 
 namespace qcs {
 
-constexpr int kTileBits = 12;               // amplitudes per MFMA tile = 4096
 constexpr int kWFloats = 64 * 64;           // W slot per block (k = 5)
 constexpr int kUDoubles = 32 * 32 * 2;      // U slot per block (k = 5)
-
-struct FusedGeom {
-  int sbits[kTileBits];  // tile bit j -> index bit (ascending)
-  int wpos[kTileBits];   // tile bit j -> LDS offset (float2 units)
-};
-
-typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 // ---- composition (FP64) -------------------------------------------------------
 
@@ -110,118 +95,6 @@ __global__ __launch_bounds__(64) void k_compose(const qcs_hip_gate *gates,
   }
 }
 
-// ---- MFMA apply (complex64, n >= 12) ------------------------------------------
-
-template <int K>
-__global__ __launch_bounds__(kThreads) void k_fused_mfma(float2 *a,
-                                                         const float *W,
-                                                         FusedGeom g,
-                                                         uint64_t ntiles) {
-  constexpr int R = 1 << K;            // complex rows
-  constexpr int RR = 2 * R;            // real rows / columns of W
-  constexpr int C = 1 << (kTileBits - K);  // columns per tile
-  constexpr int CS = C + 1;            // LDS row stride (float2), padded
-  constexpr int MT = (RR + 31) / 32;   // 32-row blocks of the output
-  constexpr int CT = C / 128;          // 32-column MFMA tiles per wave
-  static_assert(CT >= 1, "at least one column tile per wave");
-  __shared__ float2 lds[R * CS];
-
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int h = lane >> 5, l32 = lane & 31;
-
-  // A operand: lane holds W[mt*32 + l32][2s + h]
-  float wa[MT][R];
-#pragma unroll
-  for (int mt = 0; mt < MT; ++mt) {
-    const int row = mt * 32 + l32;
-#pragma unroll
-    for (int s = 0; s < R; ++s)
-      wa[mt][s] = (row < RR) ? W[row * RR + 2 * s + h] : 0.0f;
-  }
-
-  // thread offsets of tile element e = 2*tid (tile bits 1..8)
-  uint64_t addr_t = 0;
-  int pos_t = 0;
-#pragma unroll
-  for (int j = 1; j <= 8; ++j)
-    if ((tid >> (j - 1)) & 1) {
-      addr_t |= uint64_t(1) << g.sbits[j];
-      pos_t += g.wpos[j];
-    }
-  const int w0 = g.wpos[0];
-  uint64_t addr_it[8];
-  int pos_it[8];
-#pragma unroll
-  for (int it = 0; it < 8; ++it) {
-    uint64_t ai = 0;
-    int pi = 0;
-#pragma unroll
-    for (int jj = 0; jj < 3; ++jj)
-      if ((it >> jj) & 1) {
-        ai |= uint64_t(1) << g.sbits[9 + jj];
-        pi += g.wpos[9 + jj];
-      }
-    addr_it[it] = ai | addr_t;
-    pos_it[it] = pi + pos_t;
-  }
-
-  for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    uint64_t base = tile;
-#pragma unroll
-    for (int j = 0; j < kTileBits; ++j) base = insert0(base, g.sbits[j]);
-
-    float4 v[8];
-#pragma unroll
-    for (int it = 0; it < 8; ++it)
-      v[it] = *reinterpret_cast<const float4 *>(a + (base | addr_it[it]));
-#pragma unroll
-    for (int it = 0; it < 8; ++it) {
-      lds[pos_it[it]] = make_float2(v[it].x, v[it].y);
-      lds[pos_it[it] + w0] = make_float2(v[it].z, v[it].w);
-    }
-    __syncthreads();
-
-    const float *ldsf = reinterpret_cast<const float *>(lds);
-#pragma unroll
-    for (int ct = 0; ct < CT; ++ct) {
-      const int col = (wave * CT + ct) * 32 + l32;
-      float b[R];
-#pragma unroll
-      for (int s = 0; s < R; ++s) b[s] = ldsf[(s * CS + col) * 2 + h];
-      floatx16 acc[MT];
-#pragma unroll
-      for (int mt = 0; mt < MT; ++mt) {
-        acc[mt] = floatx16{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f,
-                           0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int s = 0; s < R; ++s)
-          acc[mt] = __builtin_amdgcn_mfma_f32_32x32x2f32(wa[mt][s], b[s],
-                                                         acc[mt], 0, 0, 0);
-      }
-#pragma unroll
-      for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int r0 = mt * 16 + 4 * q + 2 * h;
-          if (r0 < R)
-            lds[r0 * CS + col] = make_float2(acc[mt][4 * q], acc[mt][4 * q + 1]);
-          if (r0 + 1 < R)
-            lds[(r0 + 1) * CS + col] =
-                make_float2(acc[mt][4 * q + 2], acc[mt][4 * q + 3]);
-        }
-    }
-    __syncthreads();
-
-#pragma unroll
-    for (int it = 0; it < 8; ++it) {
-      const float2 x = lds[pos_it[it]], y = lds[pos_it[it] + w0];
-      *reinterpret_cast<float4 *>(a + (base | addr_it[it])) =
-          make_float4(x.x, x.y, y.x, y.y);
-    }
-    __syncthreads();
-  }
-}
-
 // ---- VALU apply (any precision, any n >= k) -------------------------------------
 
 template <typename T, int K>
@@ -281,38 +154,6 @@ static int grow(void **p, size_t *cap, size_t need) {
   return 0;
 }
 
-static FusedGeom make_geom(int n, int k, const int *targets) {
-  FusedGeom g;
-  const int lc = kTileBits - k;
-  const int cs = (1 << lc) + 1;
-  int is_t[64] = {0};
-  for (int q = 0; q < k; ++q) is_t[targets[q]] = 1;
-  int nl = 0, j = 0;
-  for (int b = 0; b < n && j < kTileBits; ++b) {
-    if (is_t[b]) {
-      int r = 0;
-      while (targets[r] != b) ++r;
-      g.sbits[j] = b;
-      g.wpos[j] = (1 << r) * cs;
-      ++j;
-    } else if (nl < lc) {
-      g.sbits[j] = b;
-      g.wpos[j] = 1 << nl;
-      ++nl;
-      ++j;
-    }
-  }
-  return g;
-}
-
-template <int K>
-static void launch_mfma_k(qcs_hip_state *s, const float *W, const FusedGeom &g,
-                          uint64_t ntiles) {
-  const uint64_t grid = ntiles < 2048 ? ntiles : 2048;
-  hipLaunchKernelGGL(k_fused_mfma<K>, dim3(unsigned(grid)), dim3(kThreads), 0,
-                     s->stream, static_cast<float2 *>(s->vec), W, g, ntiles);
-}
-
 template <typename T, int K>
 static void launch_valu_k(qcs_hip_state *s, const double *U, const int *t) {
   const uint64_t ncols = s->size >> K;
@@ -344,17 +185,8 @@ static int apply_block(qcs_hip_state *s, int k, const int *targets,
   TimedLaunch tl;
   int rc;
   timing_begin(s, QCS_HIP_K_FUSED, bytes, flops, &tl);
-  if (s->precision == QCS_HIP_C64 && s->n >= kTileBits) {
-    const FusedGeom g = make_geom(s->n, k, targets);
-    const uint64_t ntiles = s->size >> kTileBits;
-    switch (k) {
-      case 1: launch_mfma_k<1>(s, W, g, ntiles); break;
-      case 2: launch_mfma_k<2>(s, W, g, ntiles); break;
-      case 3: launch_mfma_k<3>(s, W, g, ntiles); break;
-      case 4: launch_mfma_k<4>(s, W, g, ntiles); break;
-      default: launch_mfma_k<5>(s, W, g, ntiles); break;
-    }
-    if ((rc = launch_check("k_fused_mfma"))) return rc;
+  if (s->precision == QCS_HIP_C64 && s->n >= 12) {
+    if ((rc = launch_block_mfma(s, k, targets, W, s->block_variant[k]))) return rc;
   } else {
     if (s->precision == QCS_HIP_C128) launch_valu<double>(s, k, U, targets);
     else launch_valu<float>(s, k, U, targets);

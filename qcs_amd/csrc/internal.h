@@ -66,6 +66,10 @@ struct qcs_hip_state {
   size_t tpass_cap = 0;
   void *tblock_dev = nullptr;
   size_t tblock_cap = 0;
+  int block_variant[QCS_HIP_MAX_FUSE + 1] = {
+      QCS_HIP_BLOCK_TB13_512_PIPE, QCS_HIP_BLOCK_TB13_512_PIPE,
+      QCS_HIP_BLOCK_TB13_512_PIPE, QCS_HIP_BLOCK_TB13_512_PIPE,
+      QCS_HIP_BLOCK_TB13_512_PIPE, QCS_HIP_BLOCK_TB13_512_PIPE};
   int tile_threads = 1024;  // work-group size of the tile-program kernel
   int tile_grid = 256;      // persistent grid of the tile-program kernel
   // kernel timing
@@ -130,6 +134,11 @@ int compose_blocks(qcs_hip_state *s, const qcs_hip_block *blocks, int nb,
 
 // kernels in tiles.hip
 int tile_max_blocks();
+
+// kernels in blockpass.hip
+int launch_block_mfma(qcs_hip_state *s, int k, const int *targets,
+                      const float *W, int variant);
+int block_variant_for(const qcs_hip_state *s, int variant);
 int tiles_apply(qcs_hip_state *s, const qcs_hip_pass *passes, int np,
                 const qcs_hip_block *blocks, int nb, const qcs_hip_gate *gates,
                 int ng);
