@@ -25,6 +25,35 @@ struct Mat2 {
   T v[8];
 };
 
+// Non-temporal (streaming) loads and stores of one amplitude.
+typedef double d2v __attribute__((ext_vector_type(2)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ double2 ld_stream(const double2 *p) {
+  const d2v v = __builtin_nontemporal_load(reinterpret_cast<const d2v *>(p));
+  return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ float2 ld_stream(const float2 *p) {
+  const f2v v = __builtin_nontemporal_load(reinterpret_cast<const f2v *>(p));
+  return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ void st_stream(double2 *p, double2 x) {
+  const d2v v = {x.x, x.y};
+  __builtin_nontemporal_store(v, reinterpret_cast<d2v *>(p));
+}
+__device__ __forceinline__ void st_stream(float2 *p, float2 x) {
+  const f2v v = {x.x, x.y};
+  __builtin_nontemporal_store(v, reinterpret_cast<f2v *>(p));
+}
+
+// XCD-aware work-group order.  The hardware places work-group b on XCD b % 8;
+// this maps b to a position so that each XCD's work-groups cover one
+// contiguous range of the grid (a bijection of [0, nb) for any nb).
+__device__ __forceinline__ uint64_t xcd_order(uint64_t b, uint64_t nb) {
+  const uint64_t x = b & 7, q = nb >> 3, r = nb & 7;
+  return x * q + (x < r ? x : r) + (b >> 3);
+}
+
 // Insert a zero bit at position b of x (pairs of a gate on qubit b).
 __device__ __forceinline__ uint64_t insert0(uint64_t x, int b) {
   const uint64_t lo = x & ((uint64_t(1) << b) - 1);
