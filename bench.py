@@ -54,7 +54,23 @@ def parse():
     p.add_argument("--no-plain", action="store_true", help="skip the plain FP64 run")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
+    p.add_argument("--copy-probe", type=float, default=0.0, metavar="GIB",
+                   help="only run a plain device copy of GIB GiB (float32, out = in * 1) "
+                        "three times: the known-size kernel that scales HBM byte "
+                        "counters (FETCH_SIZE, WRITE_SIZE)")
     return p.parse_args()
+
+
+def copy_probe(gib):
+    import torch
+    n = int(gib * 2**30) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n,
+                      "launches": 3}), flush=True)
 
 
 def timing(hip, state, cls):
@@ -133,6 +149,9 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
               "the state-vector path here is single-GPU by design", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     import qcs_amd
     from qcs_amd import circuits as C
 
