@@ -100,6 +100,11 @@ void qc_swap(t_q_circuit *circuit, int qubit_a, int qubit_b);
 #define QCS_FUSE_K(k) ((k) << 8) /* fused form: max qubits per fused block,
                                     1..5 (default 5; at most 4 with
                                     QCS_FUSE_TILES) */
+#define QCS_DEFERRED 0x20       /* no device state: gate calls are only
+                                   recorded and the circuit runs in a batch
+                                   (qc_batch_*), complex128 gate by gate or,
+                                   with QCS_FUSED_FP32, fused complex64;
+                                   1..22 qubits */
 t_q_circuit *qc_create_ex(int num_qubits, int options);
 int qc_get_options(t_q_circuit *circuit);
 
@@ -140,5 +145,47 @@ long qc_get_num_fused_passes(t_q_circuit *circuit);
 /* The circuit's device state (a qcs_hip_state *, see qcs_hip.h), after
  * applying every recorded gate; NULL when the circuit has no state. */
 void *qc_get_device_state(t_q_circuit *circuit);
+
+/* Batched execution of QCS_DEFERRED circuits: every circuit starts from
+ * |0...0>, runs its recorded gates and is sampled, all in one persistent
+ * device launch (small states stay in on-chip memory).  A complex128 circuit
+ * gives the amplitudes and samples of qc_create + the same gates +
+ * qc_sample, bit for bit; a QCS_FUSED_FP32 one those of the fused form
+ * within its tolerance.  Not thread-safe, like the rest of the library. */
+typedef struct s_q_batch t_q_batch;
+#define QCS_BATCH_GRAPH 0x1       /* capture the launch sequence as a graph
+                                     and replay it while the batch's shape
+                                     (counts and sizes) is unchanged */
+#define QCS_BATCH_KEEP_STATES 0x2 /* keep the final states for
+                                     qc_batch_get_amplitudes */
+t_q_batch *qc_batch_create(int options);
+void qc_batch_destroy(t_q_batch *batch);
+/* indices[i * shots + s]: shot s of circuits[i] (-1: dropped).  Uniforms
+ * come from rand() circuit after circuit, shot after shot -- the stream
+ * qc_sample on each circuit in turn would use.  Returns 0 or a negative
+ * QCS_HIP_E* code (see qcs_hip.h). */
+int qc_batch_sample(t_q_batch *batch, t_q_circuit **circuits, int count,
+                    int shots, long *indices);
+/* qc_run_shots for every circuit: results[i] holds 2^num_qubits ints of
+ * circuits[i], zeroed here. */
+int qc_batch_run_shots(t_q_batch *batch, t_q_circuit **circuits, int count,
+                       int shots, int **results);
+/* 2*count doubles {re, im} of circuit i's final state in the last run
+ * (QCS_BATCH_KEEP_STATES). */
+int qc_batch_get_amplitudes(t_q_batch *batch, int circuit, long offset,
+                            long count, double *out);
+/* Device time of the last run in ms: the batch kernel (-1 when the run
+ * replayed a graph) and the whole launch sequence.  Returns 1 when the last
+ * run replayed a captured graph, else 0. */
+int qc_batch_last_timing(t_q_batch *batch, double *kernel_ms,
+                         double *total_ms);
+/* Plan of the circuits with `num_qubits` qubits: LDS tile bits (complex128
+ * <= 13, complex64 <= 14; a state of at most that many qubits stays on
+ * chip) and, for fused circuits, the fusion width 1..5.  0 keeps the
+ * default (the largest tile; the circuit's QCS_FUSE_K or 5). */
+int qc_batch_set_plan(t_q_batch *batch, int fused, int num_qubits,
+                      int tile_bits, int fuse_k);
+/* threads per work-group of the batch kernel: 256 or 512 (default). */
+int qc_batch_set_threads(t_q_batch *batch, int threads);
 
 #endif

@@ -237,6 +237,68 @@ int qcs_hip_timing_enable(qcs_hip_state *s, int enable);
 int qcs_hip_timing_read(qcs_hip_state *s, int kernel_class, double *total_ms,
                         long *launches, double *bytes, double *flops);
 
+/* ---- batched circuits ----------------------------------------------------
+ * Many independent small circuits in one persistent launch.  Each circuit
+ * starts from |0...0>, runs its gates (textbook semantics) and draws
+ * `shots` samples exactly as qcs_hip_sample would on a state of its own.
+ * A work-group takes the next circuit from a device queue (largest first),
+ * keeps the state in LDS when it fits (n <= tile_bits) and otherwise
+ * streams it through LDS tile by tile, one pass per entry of its pass list.
+ *   QCS_HIP_C128: the gates are applied one by one with the arithmetic of
+ *     qcs_hip_apply_1q / qcs_hip_apply_cu / qcs_hip_apply_swap, so the final
+ *     amplitudes and samples equal those of a qcs_hip_state of its own bit
+ *     for bit.  A pass's first_block / num_blocks count gates; a pass is a
+ *     run of consecutive gates whose qubits lie in the pass's bits.
+ *   QCS_HIP_C64: fused precision form.  A pass's blocks (composed on the
+ *     device in FP64 like qcs_hip_apply_fused) are applied on the FP32
+ *     matrix cores; the state is then renormalised in FP64 (norm_sq > 1e-12
+ *     and != 1) before sampling.
+ * Pass bits are ascending (complex64: including bit 0; tiles that begin
+ * with bits 0..5 read HBM in >= 1 KiB runs); a circuit with
+ * n <= tile_bits has tile_bits == n.  No reference counterpart: the
+ * reference runs one circuit at a time (qc_run_shots, src/qcs.c:575-607). */
+#define QCS_HIP_BATCH_MAX_QUBITS 22
+#define QCS_HIP_BATCH_C128_TILE_BITS 13 /* largest LDS tile, complex128 */
+#define QCS_HIP_BATCH_C64_TILE_BITS 14  /* largest LDS tile, complex64 */
+typedef struct qcs_hip_batch qcs_hip_batch;
+typedef struct qcs_hip_batch_circuit {
+  int num_qubits;  /* 1 .. QCS_HIP_BATCH_MAX_QUBITS */
+  int precision;   /* QCS_HIP_C128 or QCS_HIP_C64 */
+  int tile_bits;   /* bits per LDS tile (== num_qubits: LDS-resident) */
+  int first_pass;  /* passes[first_pass .. first_pass + num_passes) */
+  int num_passes;  /* >= 1 */
+  int reserved[3];
+} qcs_hip_batch_circuit;
+/* flags of qcs_hip_batch_create */
+#define QCS_HIP_BATCH_KEEP_STATES 1 /* keep every final state in HBM for
+                                       qcs_hip_batch_read_state */
+#define QCS_HIP_BATCH_GRAPH 2       /* capture the launch sequence as a
+                                       hipGraph and replay it while the
+                                       batch's shape is unchanged */
+int qcs_hip_batch_create(qcs_hip_batch **out, int flags);
+void qcs_hip_batch_destroy(qcs_hip_batch *b);
+/* QCS_HIP_TUNE_BATCH_THREADS: 256 or 512 threads per work-group
+ * (default 512). */
+#define QCS_HIP_TUNE_BATCH_THREADS 0
+int qcs_hip_batch_set_tuning(qcs_hip_batch *b, int knob, int value);
+/* Run `num_circuits` circuits; u holds num_circuits * shots uniforms
+ * (circuit-major, the order the host's rand() stream would feed
+ * qc_run_shots circuit after circuit) and index_out receives the sampled
+ * basis index of every shot (-1: dropped, as qcs_hip_sample). */
+int qcs_hip_batch_run(qcs_hip_batch *b, const qcs_hip_batch_circuit *circuits,
+                      int num_circuits, const qcs_hip_pass *passes,
+                      int num_passes, const qcs_hip_block *blocks,
+                      int num_blocks, const qcs_hip_gate *gates, int num_gates,
+                      long shots, const double *u, long *index_out);
+/* amplitudes [offset, offset + count) of circuit i's final state after the
+ * last run (QCS_HIP_BATCH_KEEP_STATES only) as {re, im} doubles */
+int qcs_hip_batch_read_state(qcs_hip_batch *b, int circuit, long offset,
+                             long count, double *out);
+/* device time of the last run's batch kernel (HIP events) and of the whole
+ * launch sequence, in ms; whether the last run replayed a captured graph */
+int qcs_hip_batch_last_timing(qcs_hip_batch *b, double *kernel_ms,
+                              double *total_ms, int *graph_replayed);
+
 #ifdef __cplusplus
 }
 #endif

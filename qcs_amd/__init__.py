@@ -10,13 +10,15 @@ import ctypes
 import numpy as np
 
 from . import circuits
-from ._native import (C64, C128, FUSE_TILES, FUSED_FP32, FUSE_K, HIP_COMPAT,
-                      K_1Q, K_COMPOSE, K_COUNT, K_CU, K_FUSED, K_OTHER,
-                      K_REDUCE, PLAIN_FP32, PLAIN_FP64, REFERENCE_COMPAT,
+from ._native import (BATCH_GRAPH, BATCH_KEEP_STATES, C64, C128, DEFERRED,
+                      FUSE_TILES, FUSED_FP32, FUSE_K, HIP_COMPAT, K_1Q,
+                      K_COMPOSE, K_COUNT, K_CU, K_FUSED, K_OTHER, K_REDUCE,
+                      PLAIN_FP32, PLAIN_FP64, REFERENCE_COMPAT,
                       RENORM_AT_BARRIER, TUNE_TILE_GRID, TUNE_TILE_THREADS,
-                      HipBlock, HipGate, HipPass, libc, load)
+                      HipBatchCircuit, HipBlock, HipGate, HipPass, libc, load)
 
-__all__ = ["Circuit", "load", "libc", "circuits", "PLAIN_FP64", "PLAIN_FP32",
+__all__ = ["Circuit", "Batch", "load", "libc", "circuits", "PLAIN_FP64", "PLAIN_FP32",
+           "DEFERRED", "BATCH_GRAPH", "BATCH_KEEP_STATES", "HipBatchCircuit",
            "FUSED_FP32", "REFERENCE_COMPAT", "RENORM_AT_BARRIER", "FUSE_K",
            "FUSE_TILES", "HipPass", "TUNE_TILE_THREADS", "TUNE_TILE_GRID",
            "C128", "C64", "HIP_COMPAT", "HipGate", "HipBlock", "K_1Q", "K_CU",
@@ -133,3 +135,81 @@ class Circuit:
         mx = ctypes.c_double(0.0)
         omf = self.lib.qc_compare(self._c, other._c, ctypes.byref(mx))
         return omf, mx.value
+
+
+class Batch:
+    """One ``t_q_batch``: runs QCS_DEFERRED circuits in one device launch."""
+
+    def __init__(self, options=0):
+        _, self.lib = load()
+        self._b = self.lib.qc_batch_create(options)
+        if not self._b:
+            raise RuntimeError(f"qc_batch_create({options:#x}) failed (no HIP device?)")
+
+    def close(self):
+        if self._b:
+            self.lib.qc_batch_destroy(self._b)
+            self._b = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _handles(circuits_):
+        arr = (ctypes.c_void_p * max(len(circuits_), 1))()
+        for i, c in enumerate(circuits_):
+            arr[i] = c.handle
+        return arr
+
+    def sample(self, circuits_, shots):
+        """int64 array (len(circuits), shots) of sampled indices."""
+        out = np.full((len(circuits_), shots), -2, np.int64)
+        rc = self.lib.qc_batch_sample(self._b, self._handles(circuits_), len(circuits_),
+                                      shots, out.ctypes.data_as(ctypes.POINTER(ctypes.c_long)))
+        if rc != 0:
+            raise RuntimeError(f"qc_batch_sample failed ({rc})")
+        return out
+
+    def run_shots(self, circuits_, shots):
+        hists = [np.full(1 << c.n, -1, np.int32) for c in circuits_]
+        ptrs = (ctypes.POINTER(ctypes.c_int) * max(len(hists), 1))()
+        for i, h in enumerate(hists):
+            ptrs[i] = h.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        rc = self.lib.qc_batch_run_shots(self._b, self._handles(circuits_), len(circuits_),
+                                         shots, ptrs)
+        if rc != 0:
+            raise RuntimeError(f"qc_batch_run_shots failed ({rc})")
+        return hists
+
+    def amplitudes(self, i, n, offset=0, count=None):
+        if count is None:
+            count = (1 << n) - offset
+        out = np.empty(2 * count, np.float64)
+        rc = self.lib.qc_batch_get_amplitudes(self._b, i, offset, count, _dptr(out))
+        if rc != 0:
+            raise RuntimeError(f"qc_batch_get_amplitudes failed ({rc})")
+        return out.view(np.complex128)
+
+    def last_timing(self):
+        k, t = ctypes.c_double(), ctypes.c_double()
+        replayed = self.lib.qc_batch_last_timing(self._b, ctypes.byref(k), ctypes.byref(t))
+        return k.value, t.value, bool(replayed)
+
+    def set_plan(self, fused, num_qubits, tile_bits=0, fuse_k=0):
+        rc = self.lib.qc_batch_set_plan(self._b, int(bool(fused)), num_qubits, tile_bits, fuse_k)
+        if rc != 0:
+            raise ValueError(f"qc_batch_set_plan rejected ({rc})")
+
+    def set_threads(self, threads):
+        rc = self.lib.qc_batch_set_threads(self._b, threads)
+        if rc != 0:
+            raise ValueError(f"qc_batch_set_threads rejected ({rc})")

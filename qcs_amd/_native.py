@@ -53,6 +53,20 @@ class HipPass(ctypes.Structure):
 TILE_BITS, TILE_LOW_BITS = 13, 6
 TUNE_TILE_THREADS, TUNE_TILE_GRID = 0, 1
 FUSE_TILES = 0x10
+DEFERRED = 0x20
+
+# batched circuits (qcs_hip.h / qcs.h)
+BATCH_MAX_QUBITS = 22
+BATCH_C128_TILE_BITS, BATCH_C64_TILE_BITS = 13, 14
+HIP_BATCH_KEEP_STATES, HIP_BATCH_GRAPH = 1, 2
+TUNE_BATCH_THREADS = 0
+BATCH_GRAPH, BATCH_KEEP_STATES = 0x1, 0x2
+
+
+class HipBatchCircuit(ctypes.Structure):
+    _fields_ = [("num_qubits", c_int), ("precision", c_int), ("tile_bits", c_int),
+                ("first_pass", c_int), ("num_passes", c_int),
+                ("reserved", c_int * 3)]
 
 _HIP_PROTOS = {
     "qcs_hip_last_error": (ctypes.c_char_p, []),
@@ -90,6 +104,14 @@ _HIP_PROTOS = {
     "qcs_hip_timing_enable": (c_int, [c_void_p, c_int]),
     "qcs_hip_timing_read": (c_int, [c_void_p, c_int, P(c_double), P(c_long),
                                     P(c_double), P(c_double)]),
+    "qcs_hip_batch_create": (c_int, [P(c_void_p), c_int]),
+    "qcs_hip_batch_destroy": (None, [c_void_p]),
+    "qcs_hip_batch_set_tuning": (c_int, [c_void_p, c_int, c_int]),
+    "qcs_hip_batch_run": (c_int, [c_void_p, P(HipBatchCircuit), c_int, P(HipPass), c_int,
+                                  P(HipBlock), c_int, P(HipGate), c_int, c_long,
+                                  P(c_double), P(c_long)]),
+    "qcs_hip_batch_read_state": (c_int, [c_void_p, c_int, c_long, c_long, P(c_double)]),
+    "qcs_hip_batch_last_timing": (c_int, [c_void_p, P(c_double), P(c_double), P(c_int)]),
 }
 
 _Q = c_void_p  # t_q_circuit *
@@ -146,6 +168,16 @@ _QCS_PROTOS = {
     "qcs_pass_plan": (c_int, [P(HipGate), c_int, c_int, c_int, c_int, c_int, c_int,
                               P(P(HipPass)), P(c_int), P(P(HipBlock)), P(c_int),
                               P(P(HipGate))]),
+    "qcs_gate_pass_plan": (c_int, [P(HipGate), c_int, c_int, c_int, c_int,
+                                   P(P(HipPass)), P(c_int)]),
+    "qc_batch_create": (c_void_p, [c_int]),
+    "qc_batch_destroy": (None, [c_void_p]),
+    "qc_batch_sample": (c_int, [c_void_p, P(_Q), c_int, c_int, P(c_long)]),
+    "qc_batch_run_shots": (c_int, [c_void_p, P(_Q), c_int, c_int, P(P(c_int))]),
+    "qc_batch_get_amplitudes": (c_int, [c_void_p, c_int, c_long, c_long, P(c_double)]),
+    "qc_batch_last_timing": (c_int, [c_void_p, P(c_double), P(c_double)]),
+    "qc_batch_set_plan": (c_int, [c_void_p, c_int, c_int, c_int, c_int]),
+    "qc_batch_set_threads": (c_int, [c_void_p, c_int]),
 }
 
 _hip = None

@@ -168,19 +168,18 @@ static int copy_out(qcs_hip_state *s, const void *dev, long off, long count,
   return 0;
 }
 
-static int validate_blocks(const qcs_hip_state *s, const qcs_hip_block *blocks,
-                           int num_blocks, const qcs_hip_gate *gates,
-                           int num_gates) {
+int validate_blocks(int n, const qcs_hip_block *blocks, int num_blocks,
+                    const qcs_hip_gate *gates, int num_gates) {
   if (num_blocks < 0 || num_gates < 0 || (num_blocks > 0 && !blocks) ||
       (num_gates > 0 && !gates))
     return fail(QCS_HIP_EINVAL, "bad block/gate arrays");
   for (int b = 0; b < num_blocks; ++b) {
     const qcs_hip_block &bl = blocks[b];
-    if (bl.k < 1 || bl.k > QCS_HIP_MAX_FUSE || bl.k > s->n)
+    if (bl.k < 1 || bl.k > QCS_HIP_MAX_FUSE || bl.k > n)
       return fail(QCS_HIP_EINVAL, "block %d: k=%d outside [1,%d]", b, bl.k,
                   QCS_HIP_MAX_FUSE);
     for (int q = 0; q < bl.k; ++q) {
-      if (bl.targets[q] < 0 || bl.targets[q] >= s->n ||
+      if (bl.targets[q] < 0 || bl.targets[q] >= n ||
           (q > 0 && bl.targets[q] <= bl.targets[q - 1]))
         return fail(QCS_HIP_EINVAL, "block %d: targets not ascending in range", b);
     }
@@ -466,7 +465,7 @@ int qcs_hip_apply_fused(qcs_hip_state *s, const qcs_hip_block *blocks,
   if ((rc = check_state(s))) return rc;
   if (s->flags & QCS_HIP_COMPAT)
     return fail(QCS_HIP_EUNSUP, "fusion needs textbook semantics");
-  if ((rc = validate_blocks(s, blocks, num_blocks, gates, num_gates))) return rc;
+  if ((rc = validate_blocks(s->n, blocks, num_blocks, gates, num_gates))) return rc;
   return fused_apply(s, blocks, num_blocks, gates, num_gates);
 }
 
@@ -530,7 +529,7 @@ int qcs_hip_apply_passes(qcs_hip_state *s, const qcs_hip_pass *passes,
   }
   if (covered != num_blocks)
     return fail(QCS_HIP_EINVAL, "passes cover %d of %d blocks", covered, num_blocks);
-  if ((rc = validate_blocks(s, blocks, num_blocks, gates, num_gates))) return rc;
+  if ((rc = validate_blocks(s->n, blocks, num_blocks, gates, num_gates))) return rc;
   return tiles_apply(s, passes, num_passes, blocks, num_blocks, gates, num_gates);
 }
 

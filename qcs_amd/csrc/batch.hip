@@ -1,0 +1,1164 @@
+// Batched circuits (qcs_hip_batch_*): thousands of small independent
+// circuits in one persistent launch.
+//
+// Work distribution.  The host orders the circuits by estimated cost,
+// largest first; each work-group of a persistent grid (as many work-groups
+// as are resident at once) takes the next circuit from a device counter, so
+// the long circuits start first and the short ones fill in behind them.
+// Every wave reaches the exit test once the counter passes the end.
+//
+// State placement.  A circuit whose 2^n amplitudes fit in the LDS tile
+// (n <= tile_bits: up to 2^13 complex128 or 2^14 complex64 amplitudes, 128
+// KiB) lives in LDS from |0...0> to its last shot; HBM sees only the
+// uniforms in and the sampled indices out.  A larger circuit keeps its state
+// in an HBM slot and streams it through LDS pass by pass (a set S of
+// tile_bits index bits with the ops on those bits) and tile by tile (the
+// 2^(n - tile_bits) tiles of amplitudes that agree outside S).  In the first
+// pass only tile 0 holds a non-zero amplitude; the others are written as
+// zeros without running the ops (every op is linear).
+//
+// complex128 (plain FP64): the ops are the circuit's gates in order, each
+// one sweep over the tile with the arithmetic and gate classes of the
+// per-gate kernels in gates.hip, so the amplitudes equal those of a state of
+// its own bit for bit.  complex64 (fused FP32): the ops are composed k-qubit
+// blocks applied on the matrix cores with the operand layout of
+// k_fused_mfma (blockpass.hip) on an XOR-swizzled tile (the layout of
+// tiles.hip, with one more fold for 14-bit tiles), then FP64
+// renormalisation.
+//
+// Sampling is qcs_hip_sample's algorithm inside one work-group: FP64 sums of
+// 4096-amplitude chunks (rows of 64, a wave prefix scan per row, rows
+// chained sequentially), a sequential scan of the chunk sums, and per shot a
+// binary search over the chunks and a scan inside the chunk -- the same
+// operations in the same order as reduce.hip, so the same state gives the
+// same indices.
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "device.h"
+#include "internal.h"
+
+namespace qcs {
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef float bf4v __attribute__((ext_vector_type(4)));
+
+constexpr int kBMaxTb = 14;
+constexpr int kBMaxN = QCS_HIP_BATCH_MAX_QUBITS;
+constexpr size_t kBAlign = 256;
+
+enum : int {
+  kOp1Dense = 0,  // 2x2 on (i0, i1)
+  kOp1Phase = 1,  // diag(1, m11): bit-1 half only
+  kOpCuDense = 2, // 2x2 on control=1 pairs
+  kOpCnot = 3,    // exchange on control=1 pairs
+  kOpCPhase = 4,  // m11 on control=1, target=1
+  kOpSwap = 5     // exchange (1,0) <-> (0,1) of qubits t, c
+};
+
+struct BCirc {
+  int n;
+  int prec;
+  int tb;
+  int npass;
+  int first_pass;
+  int reserved;
+  uint64_t state_off;  // byte offset of the HBM slot, kNoSlot when none
+  uint64_t shot_off;   // first shot in u / out
+};
+constexpr uint64_t kNoSlot = ~uint64_t(0);
+
+struct BPass {
+  int first_op;
+  int nops;
+  int sbits[kBMaxTb];  // tile bit j -> index bit, ascending
+};
+
+struct BGate {
+  int cls;
+  int t;  // tile positions
+  int c;
+  int reserved;
+  double m[8];
+};
+
+struct BBlock {
+  int k;
+  int wslot;
+  int tpos[6];        // tile positions of the targets, ascending
+  int rowslot[32];    // byte offset 8 * bswz(row bits) of row s
+};
+
+// XOR swizzle of a complex64 tile element (float2 slot); linear over GF(2)
+// and a bijection of every [0, 2^tb), tb >= 5.
+__host__ __device__ __forceinline__ int bswz(int e) {
+  return e ^ ((e >> 5) & 31) ^ ((e >> 10) & 15);
+}
+
+__device__ __forceinline__ int bins0(int x, int b) {
+  return ((x >> b) << (b + 1)) | (x & ((1 << b) - 1));
+}
+
+// first index of tile t: t's bits spread over the index bits outside S
+__device__ __forceinline__ uint64_t tile_base(uint64_t t, const int *sb,
+                                              int tb) {
+  for (int j = 0; j < tb; ++j) t = insert0(t, sb[j]);
+  return t;
+}
+
+// index bits of tile element e: bit j of e goes to index bit sb[j], for
+// j in [j0, tb)
+__device__ __forceinline__ uint64_t deposit(uint32_t e, const int *sb, int j0,
+                                            int tb) {
+  uint64_t a = 0;
+  for (int j = j0; j < tb; ++j)
+    if ((e >> j) & 1u) a |= uint64_t(1) << sb[j];
+  return a;
+}
+
+// ---- complex128: gates on an LDS tile -------------------------------------------
+
+template <int NTH>
+__device__ void tile_gate(double2 *L, int tb, const BGate *__restrict__ gp) {
+  const int cls = gp->cls, t = gp->t, c = gp->c;
+  double m[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) m[i] = gp->m[i];
+  const int tid = threadIdx.x;
+  if (cls == kOp1Dense || cls == kOp1Phase) {
+    const int tbit = 1 << t, np = 1 << (tb - 1);
+    if (cls == kOp1Dense) {
+      for (int p = tid; p < np; p += NTH) {
+        const int i0 = bins0(p, t), i1 = i0 | tbit;
+        const double2 v0 = L[i0], v1 = L[i1];
+        double2 o0, o1;
+        row2(m[0], m[1], m[2], m[3], v0.x, v0.y, v1.x, v1.y, o0.x, o0.y);
+        row2(m[4], m[5], m[6], m[7], v0.x, v0.y, v1.x, v1.y, o1.x, o1.y);
+        L[i0] = o0;
+        L[i1] = o1;
+      }
+    } else {
+      for (int p = tid; p < np; p += NTH) {
+        const int i1 = bins0(p, t) | tbit;
+        const double2 v = L[i1];
+        double2 o;
+        cmul(m[6], m[7], v.x, v.y, o.x, o.y);
+        L[i1] = o;
+      }
+    }
+    return;
+  }
+  const int lo = t < c ? t : c, hi = t < c ? c : t;
+  const int nq = 1 << (tb - 2);
+  if (cls == kOpSwap) {
+    const int bl = 1 << lo, bh = 1 << hi;
+    for (int q = tid; q < nq; q += NTH) {
+      const int i = bins0(bins0(q, lo), hi);
+      const double2 x = L[i | bl], y = L[i | bh];
+      L[i | bl] = y;
+      L[i | bh] = x;
+    }
+    return;
+  }
+  const int cb = 1 << c, tbit = 1 << t;
+  for (int q = tid; q < nq; q += NTH) {
+    const int i0 = bins0(bins0(q, lo), hi) | cb, i1 = i0 | tbit;
+    if (cls == kOpCuDense) {
+      const double2 v0 = L[i0], v1 = L[i1];
+      double2 o0, o1;
+      row2(m[0], m[1], m[2], m[3], v0.x, v0.y, v1.x, v1.y, o0.x, o0.y);
+      row2(m[4], m[5], m[6], m[7], v0.x, v0.y, v1.x, v1.y, o1.x, o1.y);
+      L[i0] = o0;
+      L[i1] = o1;
+    } else if (cls == kOpCnot) {
+      const double2 v0 = L[i0], v1 = L[i1];
+      L[i0] = v1;
+      L[i1] = v0;
+    } else {
+      const double2 v = L[i1];
+      double2 o;
+      cmul(m[6], m[7], v.x, v.y, o.x, o.y);
+      L[i1] = o;
+    }
+  }
+}
+
+// complex128 tile <-> HBM: element e <-> G[base | deposit(e)]
+template <int NTH, bool LOAD>
+__device__ void tile_io_c128(double2 *L, double2 *G, uint64_t base,
+                             const int *sb, int tb) {
+  constexpr int LT = __builtin_ctz(NTH), U = 4;
+  const int tid = threadIdx.x, tile = 1 << tb;
+  const uint64_t dt = base | deposit(uint32_t(tid), sb, 0, tb < LT ? tb : LT);
+  for (int e0 = 0; e0 < tile; e0 += U * NTH) {
+    double2 v[U];
+    uint64_t a[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e = e0 + u * NTH + tid;
+      a[u] = dt | deposit(uint32_t(e), sb, LT, tb);
+      if (LOAD && e < tile) v[u] = G[a[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e = e0 + u * NTH + tid;
+      if (e >= tile) continue;
+      if (LOAD) L[e] = v[u];
+      else G[a[u]] = L[e];
+    }
+  }
+}
+
+// ---- complex64: fused blocks on an LDS tile --------------------------------------
+
+// One composed k-qubit block on the swizzled tile: rows are the 2^k patterns
+// of the target bits, columns the other tile bits; each wave takes 32
+// columns at a time, W (A operand) in registers, the tile column as B.
+template <int K, int NTH>
+__device__ void tile_block(char *Lb, int tb, const BBlock *__restrict__ bp,
+                           const float *__restrict__ wbuf) {
+  constexpr int R = 1 << K, RR = 2 * R, MT = (RR + 31) / 32, NW = NTH / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int h = lane >> 5, l32 = lane & 31;
+  const float *W = wbuf + size_t(bp->wslot) * kWSlotFloats;
+  float wa[MT][R];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) {
+    const int row = mt * 32 + l32;
+#pragma unroll
+    for (int s = 0; s < R; ++s)
+      wa[mt][s] = (row < RR) ? W[row * RR + 2 * s + h] : 0.0f;
+  }
+  int tpos[K], rs[R];
+#pragma unroll
+  for (int q = 0; q < K; ++q) tpos[q] = bp->tpos[q];
+#pragma unroll
+  for (int s = 0; s < R; ++s) rs[s] = bp->rowslot[s];
+  const int ncols = 1 << (tb - K);
+  for (int c0 = wave * 32; c0 < ncols; c0 += NW * 32) {
+    const int cc = c0 + l32;
+    const bool ok = cc < ncols;
+    int col = ok ? cc : 0;
+#pragma unroll
+    for (int q = 0; q < K; ++q) col = bins0(col, tpos[q]);
+    const int cs = 8 * bswz(col), csh = cs + 4 * h;
+    float b[R];
+#pragma unroll
+    for (int s = 0; s < R; ++s)
+      b[s] = ok ? *reinterpret_cast<const float *>(Lb + (csh ^ rs[s])) : 0.0f;
+    floatx16 acc[MT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+      acc[mt] = floatx16{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f,
+                         0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int s = 0; s < R; ++s)
+        acc[mt] = __builtin_amdgcn_mfma_f32_32x32x2f32(wa[mt][s], b[s], acc[mt], 0, 0, 0);
+    }
+    if (!ok) continue;
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r0 = mt * 16 + 4 * q + 2 * h;
+        if (r0 < R)
+          *reinterpret_cast<float2 *>(Lb + (cs ^ rs[r0])) =
+              make_float2(acc[mt][4 * q], acc[mt][4 * q + 1]);
+        if (r0 + 1 < R)
+          *reinterpret_cast<float2 *>(Lb + (cs ^ rs[r0 + 1])) =
+              make_float2(acc[mt][4 * q + 2], acc[mt][4 * q + 3]);
+      }
+  }
+}
+
+template <int NTH>
+__device__ void tile_block_any(char *Lb, int tb, const BBlock *__restrict__ bp,
+                               const float *__restrict__ wbuf) {
+  switch (bp->k) {
+    case 1: tile_block<1, NTH>(Lb, tb, bp, wbuf); break;
+    case 2: tile_block<2, NTH>(Lb, tb, bp, wbuf); break;
+    case 3: tile_block<3, NTH>(Lb, tb, bp, wbuf); break;
+    case 4: tile_block<4, NTH>(Lb, tb, bp, wbuf); break;
+    default: tile_block<5, NTH>(Lb, tb, bp, wbuf); break;
+  }
+}
+
+// complex64 tile <-> HBM in element pairs (e, e + 1), e even: tile bit 0 is
+// index bit 0, so a pair is one 16-byte access, and swizzled slots
+// bswz(e), bswz(e) ^ 1.
+template <int NTH, bool LOAD>
+__device__ void tile_io_c64(char *Lb, float2 *G, uint64_t base, const int *sb,
+                            int tb) {
+  constexpr int LT = __builtin_ctz(NTH), U = 4;
+  const int tid = threadIdx.x, npair = 1 << (tb - 1);
+  const int jt = tb - 1 < LT ? tb - 1 : LT;  // tile bits 1..jt from tid
+  const uint64_t dt = base | deposit(uint32_t(tid) << 1, sb, 1, 1 + jt);
+  for (int p0 = 0; p0 < npair; p0 += U * NTH) {
+    bf4v v[U];
+    uint64_t a[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int p = p0 + u * NTH + tid;
+      a[u] = dt | deposit(uint32_t(p) << 1, sb, 1 + LT, tb);
+      if (LOAD && p < npair)
+        v[u] = *reinterpret_cast<const bf4v *>(G + a[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int p = p0 + u * NTH + tid;
+      if (p >= npair) continue;
+      float2 *s0 = reinterpret_cast<float2 *>(Lb + 8 * bswz(2 * p));
+      float2 *s1 = reinterpret_cast<float2 *>(Lb + 8 * (bswz(2 * p) ^ 1));
+      if (LOAD) {
+        *s0 = make_float2(v[u].x, v[u].y);
+        *s1 = make_float2(v[u].z, v[u].w);
+      } else {
+        const float2 x = *s0, y = *s1;
+        const bf4v w = {x.x, x.y, y.x, y.y};
+        *reinterpret_cast<bf4v *>(G + a[u]) = w;
+      }
+    }
+  }
+}
+
+// ---- FP64 norm and sampling ------------------------------------------------------
+
+// Sum over the work-group, broadcast to every thread.
+template <int NTH>
+__device__ double wg_sum(double v, double *red) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double r = 0.0;
+  for (int w = 0; w < NTH / 64; ++w) r += red[w];
+  __syncthreads();
+  return r;
+}
+
+// qcs_hip_sample on one state; prob(i) = |a_i|^2 in FP64.
+template <int NTH, class P>
+__device__ void sample_state(const P &prob, int n, double *tot, double *inc,
+                             const double *__restrict__ u,
+                             long *__restrict__ out, uint64_t shots) {
+  constexpr int NW = NTH / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t N = uint64_t(1) << n;
+  const uint64_t len = N < uint64_t(kChunk) ? N : uint64_t(kChunk);
+  const uint64_t nch = N / len;
+  for (uint64_t ch = wave; ch < nch; ch += NW) {
+    double running = 0.0;
+    for (uint64_t r = 0; r < len; r += 64) {
+      const double x = (r + lane < len) ? prob(ch * len + r + lane) : 0.0;
+      const double s = wave_incl_scan(x, lane);
+      running = running + __shfl(s, 63, 64);
+    }
+    if (lane == 0) tot[ch] = running;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double run = 0.0;
+    for (uint64_t ch = 0; ch < nch; ++ch) {
+      const double t = tot[ch];
+      tot[ch] = run;
+      run = run + t;
+      inc[ch] = run;
+    }
+  }
+  __syncthreads();
+  for (uint64_t s = wave; s < shots; s += NW) {
+    const double us = u[s];
+    uint64_t lo = 0, hi = nch;
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (inc[mid] > us) hi = mid; else lo = mid + 1;
+    }
+    if (lo >= nch) {
+      if (lane == 0) out[s] = -1;
+      continue;
+    }
+    const uint64_t c = lo;
+    const double e = tot[c];
+    double running = 0.0;
+    long found = -1, last_nz = -1;
+    for (uint64_t r = 0; r < len; r += 64) {
+      const double x = (r + lane < len) ? prob(c * len + r + lane) : 0.0;
+      const double sc = wave_incl_scan(x, lane);
+      const double local = running + sc;
+      const double cdf = e + local;
+      const unsigned long long hit = __ballot(cdf > us && x > 0.0);
+      const unsigned long long nz = __ballot(x > 0.0);
+      if (nz) last_nz = long(c * len + r) + 63 - __clzll(nz);
+      if (hit) {
+        found = long(c * len + r) + __ffsll(hit) - 1;
+        break;
+      }
+      running = running + __shfl(sc, 63, 64);
+    }
+    if (found < 0) found = last_nz;
+    if (lane == 0) out[s] = found;
+  }
+  __syncthreads();
+}
+
+// ---- one circuit per work-group ---------------------------------------------------
+
+template <int NTH>
+__device__ void run_c128(const BCirc &c, const BPass *__restrict__ passes,
+                         const BGate *__restrict__ gates, char *states,
+                         char *lds, double *tot, double *inc, double *red,
+                         const double *__restrict__ u, long *__restrict__ out,
+                         uint64_t shots, int keep) {
+  double2 *L = reinterpret_cast<double2 *>(lds);
+  const int n = c.n, tb = c.tb, tile = 1 << tb, tid = threadIdx.x;
+  const bool resident = tb == n;
+  double2 *G = c.state_off == kNoSlot
+                   ? nullptr
+                   : reinterpret_cast<double2 *>(states + c.state_off);
+  const uint64_t ntiles = uint64_t(1) << (n - tb);
+  for (int p = 0; p < c.npass; ++p) {
+    const BPass &ps = passes[c.first_pass + p];
+    for (uint64_t t = 0; t < ntiles; ++t) {
+      const uint64_t base = tile_base(t, ps.sbits, tb);
+      if (p == 0 || resident) {
+        if (p == 0)
+          for (int e = tid; e < tile; e += NTH)
+            L[e] = make_double2((t == 0 && e == 0) ? 1.0 : 0.0, 0.0);
+      } else {
+        tile_io_c128<NTH, true>(L, G, base, ps.sbits, tb);
+      }
+      __syncthreads();
+      if (p > 0 || t == 0) {
+        for (int o = 0; o < ps.nops; ++o) {
+          tile_gate<NTH>(L, tb, &gates[ps.first_op + o]);
+          __syncthreads();
+        }
+      }
+      if (!resident) {
+        tile_io_c128<NTH, false>(L, G, base, ps.sbits, tb);
+        __syncthreads();
+      }
+    }
+  }
+  if (resident && keep && G) {
+    for (int e = tid; e < tile; e += NTH) G[e] = L[e];
+    __syncthreads();
+  }
+  (void)red;
+  if (shots == 0) return;
+  if (resident) {
+    sample_state<NTH>([&](uint64_t i) { return norm2(L[i]); }, n, tot, inc,
+                      u + c.shot_off, out + c.shot_off, shots);
+  } else {
+    const double2 *Gc = G;
+    sample_state<NTH>([&](uint64_t i) { return norm2(Gc[i]); }, n, tot, inc,
+                      u + c.shot_off, out + c.shot_off, shots);
+  }
+}
+
+template <int NTH>
+__device__ void run_c64(const BCirc &c, const BPass *__restrict__ passes,
+                        const BBlock *__restrict__ blocks,
+                        const float *__restrict__ wbuf, char *states, char *lds,
+                        double *tot, double *inc, double *red,
+                        const double *__restrict__ u, long *__restrict__ out,
+                        uint64_t shots, int keep) {
+  const int n = c.n, tb = c.tb, tile = 1 << tb, tid = threadIdx.x;
+  const bool resident = tb == n;
+  float2 *L = reinterpret_cast<float2 *>(lds);
+  float2 *G = c.state_off == kNoSlot
+                  ? nullptr
+                  : reinterpret_cast<float2 *>(states + c.state_off);
+  const uint64_t ntiles = uint64_t(1) << (n - tb);
+  // tiles of fewer than 32 elements are not swizzled (bswz needs tb >= 5)
+  for (int p = 0; p < c.npass; ++p) {
+    const BPass &ps = passes[c.first_pass + p];
+    for (uint64_t t = 0; t < ntiles; ++t) {
+      const uint64_t base = tile_base(t, ps.sbits, tb);
+      if (p == 0 || resident) {
+        if (p == 0)
+          for (int e = tid; e < tile; e += NTH)
+            L[e] = make_float2((t == 0 && e == 0) ? 1.0f : 0.0f, 0.0f);
+      } else {
+        tile_io_c64<NTH, true>(lds, G, base, ps.sbits, tb);
+      }
+      __syncthreads();
+      if (p > 0 || t == 0) {
+        for (int o = 0; o < ps.nops; ++o) {
+          tile_block_any<NTH>(lds, tb, &blocks[ps.first_op + o], wbuf);
+          __syncthreads();
+        }
+      }
+      if (!resident) {
+        tile_io_c64<NTH, false>(lds, G, base, ps.sbits, tb);
+        __syncthreads();
+      }
+    }
+  }
+  // FP64 renormalisation, q_state_normalize's rule (src/q_utils.c:112-118)
+  const uint64_t N = uint64_t(1) << n;
+  double ns = 0.0;
+  if (resident)
+    for (int e = tid; e < tile; e += NTH) ns += norm2(L[e]);
+  else
+    for (uint64_t i = tid; i < N; i += NTH) ns += norm2(G[i]);
+  ns = wg_sum<NTH>(ns, red);
+  if (ns > 1e-12 && ns != 1.0) {
+    const double f = 1.0 / sqrt(ns);
+    if (resident) {
+      for (int e = tid; e < tile; e += NTH) {
+        float2 x = L[e];
+        x.x = float(double(x.x) * f);
+        x.y = float(double(x.y) * f);
+        L[e] = x;
+      }
+    } else {
+      for (uint64_t i = tid; i < N; i += NTH) {
+        float2 x = G[i];
+        x.x = float(double(x.x) * f);
+        x.y = float(double(x.y) * f);
+        G[i] = x;
+      }
+    }
+    __syncthreads();
+  }
+  if (resident && keep && G) {
+    for (int e = tid; e < tile; e += NTH) G[e] = L[bswz(e)];
+    __syncthreads();
+  }
+  if (shots == 0) return;
+  if (resident) {
+    sample_state<NTH>([&](uint64_t i) { return norm2(L[bswz(int(i))]); }, n,
+                      tot, inc, u + c.shot_off, out + c.shot_off, shots);
+  } else {
+    const float2 *Gc = G;
+    sample_state<NTH>([&](uint64_t i) { return norm2(Gc[i]); }, n, tot, inc,
+                      u + c.shot_off, out + c.shot_off, shots);
+  }
+}
+
+// The persistent kernel.  `queue` starts at 0 (k_batch_reset); each
+// work-group takes circuits until the counter passes nc.
+template <int NTH>
+__global__ __launch_bounds__(NTH) void k_batch(
+    const BCirc *__restrict__ circs, const int *__restrict__ order, int nc,
+    const BPass *__restrict__ passes, const BGate *__restrict__ gates,
+    const BBlock *__restrict__ blocks, const float *__restrict__ wbuf,
+    char *states, const double *__restrict__ u, long *__restrict__ out,
+    uint64_t shots, unsigned *queue, int tile_bytes, int nch_cap, int keep) {
+  extern __shared__ double4 smem4[];
+  char *lds = reinterpret_cast<char *>(smem4);
+  double *tot = reinterpret_cast<double *>(lds + tile_bytes);
+  double *inc = tot + nch_cap;
+  __shared__ double red[NTH / 64];
+  __shared__ int next;
+  for (;;) {
+    if (threadIdx.x == 0) next = int(atomicAdd(queue, 1u));
+    __syncthreads();
+    const int i = __builtin_amdgcn_readfirstlane(next);
+    __syncthreads();
+    if (i >= nc) break;
+    const BCirc c = circs[order[i]];
+    if (c.prec == QCS_HIP_C128)
+      run_c128<NTH>(c, passes, gates, states, lds, tot, inc, red, u, out,
+                    shots, keep);
+    else
+      run_c64<NTH>(c, passes, blocks, wbuf, states, lds, tot, inc, red, u,
+                   out, shots, keep);
+  }
+}
+
+__global__ void k_batch_reset(unsigned *queue) {
+  if (threadIdx.x == 0) queue[0] = 0u;
+}
+
+}  // namespace qcs
+
+// ---- host side ---------------------------------------------------------------------
+
+struct qcs_hip_batch {
+  int flags = 0;
+  int device = 0;
+  int nth = 512;
+  int num_cus = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  // device buffers, grown on demand
+  void *d_desc = nullptr;  // descriptors, compose inputs and uniforms
+  size_t desc_cap = 0;
+  float *d_w = nullptr;
+  size_t w_cap = 0;  // blocks
+  char *d_states = nullptr;
+  size_t states_cap = 0;
+  long *d_out = nullptr;
+  size_t out_cap = 0;
+  unsigned *d_queue = nullptr;
+  // pinned staging (same layout as d_desc) and output
+  char *h_stage = nullptr;
+  size_t stage_cap = 0;
+  long *h_out = nullptr;
+  size_t hout_cap = 0;
+  long generation = 0;  // bumped whenever a buffer moves
+  // captured launch sequence
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<long> graph_key;
+  // last run, for read_state and timing
+  std::vector<int> last_n, last_prec;
+  std::vector<uint64_t> last_off;
+  float kernel_ms = -1.0f, total_ms = -1.0f;
+  int replayed = 0;
+};
+
+namespace qcs {
+
+struct BatchLayout {
+  size_t circ, order, pass, gate, block, hgate, hblock, u, total;
+};
+
+static size_t align_up(size_t x) { return (x + kBAlign - 1) / kBAlign * kBAlign; }
+
+static void drop_graph(qcs_hip_batch *b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  b->graph_key.clear();
+}
+
+static int grow_device(qcs_hip_batch *b, void **p, size_t *cap, size_t need) {
+  if (*cap >= need) return 0;
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  *cap = 0;
+  ++b->generation;
+  QCS_HIP_TRY(hipMalloc(p, need));
+  *cap = need;
+  return 0;
+}
+
+static int grow_pinned(qcs_hip_batch *b, void **p, size_t *cap, size_t need) {
+  if (*cap >= need) return 0;
+  if (*p) (void)hipHostFree(*p);
+  *p = nullptr;
+  *cap = 0;
+  ++b->generation;
+  QCS_HIP_TRY(hipHostMalloc(p, need, hipHostMallocDefault));
+  *cap = need;
+  return 0;
+}
+
+static int max_tb(int prec) {
+  return prec == QCS_HIP_C128 ? QCS_HIP_BATCH_C128_TILE_BITS
+                              : QCS_HIP_BATCH_C64_TILE_BITS;
+}
+
+// every argument check of qcs_hip_batch_run; fills the op counts
+static int validate_batch(const qcs_hip_batch_circuit *cs, int nc,
+                          const qcs_hip_pass *passes, int np,
+                          const qcs_hip_block *blocks, int nb,
+                          const qcs_hip_gate *gates, int ng, long shots,
+                          const double *u, long *out, size_t *n_bgate,
+                          size_t *n_bblock) {
+  if (nc < 0 || np < 0 || nb < 0 || ng < 0 || shots < 0)
+    return fail(QCS_HIP_EINVAL, "negative count");
+  if ((nc > 0 && !cs) || (np > 0 && !passes) || (nb > 0 && !blocks) ||
+      (ng > 0 && !gates))
+    return fail(QCS_HIP_EINVAL, "null array");
+  if (shots > 0 && nc > 0 && (!u || !out))
+    return fail(QCS_HIP_EINVAL, "null uniforms or output");
+  if (shots > (long(1) << 40) / (nc > 0 ? nc : 1))
+    return fail(QCS_HIP_EINVAL, "too many shots");
+  *n_bgate = *n_bblock = 0;
+  for (int i = 0; i < nc; ++i) {
+    const qcs_hip_batch_circuit &c = cs[i];
+    if (c.num_qubits < 1 || c.num_qubits > kBMaxN)
+      return fail(QCS_HIP_EINVAL, "circuit %d: num_qubits %d outside [1,%d]", i,
+                  c.num_qubits, kBMaxN);
+    if (c.precision != QCS_HIP_C128 && c.precision != QCS_HIP_C64)
+      return fail(QCS_HIP_EINVAL, "circuit %d: unknown precision %d", i, c.precision);
+    const int n = c.num_qubits, tb = c.tile_bits;
+    if (tb < 1 || tb > n || tb > max_tb(c.precision))
+      return fail(QCS_HIP_EINVAL, "circuit %d: tile_bits %d outside [1,%d]", i, tb,
+                  std::min(n, max_tb(c.precision)));
+    if (c.precision == QCS_HIP_C64 && tb < 5 && tb != n)
+      return fail(QCS_HIP_EINVAL, "circuit %d: complex64 tiles of < 5 bits must hold the state", i);
+    if (c.num_passes < 1 || c.first_pass < 0 || c.first_pass + c.num_passes > np)
+      return fail(QCS_HIP_EINVAL, "circuit %d: pass range outside the pass array", i);
+    for (int p = c.first_pass; p < c.first_pass + c.num_passes; ++p) {
+      const qcs_hip_pass &ps = passes[p];
+      if (ps.tile_bits != tb)
+        return fail(QCS_HIP_EINVAL, "pass %d: tile_bits %d != circuit's %d", p,
+                    ps.tile_bits, tb);
+      for (int j = 0; j < tb; ++j) {
+        if (ps.bits[j] < 0 || ps.bits[j] >= n || (j > 0 && ps.bits[j] <= ps.bits[j - 1]))
+          return fail(QCS_HIP_EINVAL, "pass %d: bits not ascending in range", p);
+      }
+      if (c.precision == QCS_HIP_C64 && ps.bits[0] != 0)
+        return fail(QCS_HIP_EINVAL, "pass %d: complex64 tiles must hold bit 0", p);
+      unsigned long long mask = 0;
+      for (int j = 0; j < tb; ++j) mask |= 1ull << ps.bits[j];
+      if (ps.first_block < 0 || ps.num_blocks < 0)
+        return fail(QCS_HIP_EINVAL, "pass %d: bad op range", p);
+      if (c.precision == QCS_HIP_C128) {
+        if (ps.first_block + ps.num_blocks > ng)
+          return fail(QCS_HIP_EINVAL, "pass %d: gate range outside the gate array", p);
+        for (int g = ps.first_block; g < ps.first_block + ps.num_blocks; ++g) {
+          const qcs_hip_gate &gt = gates[g];
+          if (gt.kind < QCS_HIP_G_1Q || gt.kind > QCS_HIP_G_SWAP || gt.target < 0 ||
+              gt.target >= n || !((mask >> gt.target) & 1))
+            return fail(QCS_HIP_EINVAL, "pass %d: gate %d outside the pass bits", p, g);
+          if (gt.kind != QCS_HIP_G_1Q &&
+              (gt.control < 0 || gt.control >= n || gt.control == gt.target ||
+               !((mask >> gt.control) & 1)))
+            return fail(QCS_HIP_EINVAL, "pass %d: gate %d: bad control", p, g);
+        }
+        *n_bgate += size_t(ps.num_blocks);
+      } else {
+        if (ps.first_block + ps.num_blocks > nb)
+          return fail(QCS_HIP_EINVAL, "pass %d: block range outside the block array", p);
+        int rc = validate_blocks(n, blocks + ps.first_block, ps.num_blocks, gates, ng);
+        if (rc) return rc;
+        for (int bi = ps.first_block; bi < ps.first_block + ps.num_blocks; ++bi) {
+          if (blocks[bi].k > tb)
+            return fail(QCS_HIP_EINVAL, "block %d: k > tile_bits", bi);
+          for (int q = 0; q < blocks[bi].k; ++q)
+            if (!((mask >> blocks[bi].targets[q]) & 1))
+              return fail(QCS_HIP_EINVAL, "block %d: target outside the pass bits", bi);
+        }
+        *n_bblock += size_t(ps.num_blocks);
+      }
+    }
+  }
+  return 0;
+}
+
+static int position_in(const qcs_hip_pass &ps, int q) {
+  for (int j = 0; j < ps.tile_bits; ++j)
+    if (ps.bits[j] == q) return j;
+  return -1;
+}
+
+static BGate make_bgate(const qcs_hip_pass &ps, const qcs_hip_gate &g) {
+  BGate o;
+  std::memset(&o, 0, sizeof(o));
+  o.t = position_in(ps, g.target);
+  o.c = g.kind == QCS_HIP_G_1Q ? 0 : position_in(ps, g.control);
+  std::memcpy(o.m, g.m, sizeof(o.m));
+  if (g.kind == QCS_HIP_G_1Q)
+    o.cls = is_phase1(g.m) ? kOp1Phase : kOp1Dense;
+  else if (g.kind == QCS_HIP_G_SWAP)
+    o.cls = kOpSwap;
+  else
+    o.cls = is_x(g.m) ? kOpCnot : (is_phase1(g.m) ? kOpCPhase : kOpCuDense);
+  return o;
+}
+
+static BBlock make_bblock(const qcs_hip_pass &ps, const qcs_hip_block &bl,
+                          int wslot, bool swizzle) {
+  BBlock o;
+  std::memset(&o, 0, sizeof(o));
+  o.k = bl.k;
+  o.wslot = wslot;
+  for (int q = 0; q < bl.k; ++q) o.tpos[q] = position_in(ps, bl.targets[q]);
+  for (int r = 0; r < (1 << bl.k); ++r) {
+    int off = 0;
+    for (int q = 0; q < bl.k; ++q)
+      if ((r >> q) & 1) off |= 1 << o.tpos[q];
+    o.rowslot[r] = 8 * (swizzle ? bswz(off) : off);
+  }
+  return o;
+}
+
+template <int NTH>
+static int launch_batch(qcs_hip_batch *b, const BatchLayout &L, int nc,
+                        uint64_t shots, int tile_bytes, int nch_cap, int grid) {
+  char *d = static_cast<char *>(b->d_desc);
+  hipLaunchKernelGGL(k_batch<NTH>, dim3(unsigned(grid)), dim3(NTH),
+                     size_t(tile_bytes) + 2 * sizeof(double) * size_t(nch_cap),
+                     b->stream, reinterpret_cast<const BCirc *>(d + L.circ),
+                     reinterpret_cast<const int *>(d + L.order), nc,
+                     reinterpret_cast<const BPass *>(d + L.pass),
+                     reinterpret_cast<const BGate *>(d + L.gate),
+                     reinterpret_cast<const BBlock *>(d + L.block), b->d_w,
+                     b->d_states, reinterpret_cast<const double *>(d + L.u),
+                     b->d_out, shots, b->d_queue, tile_bytes, nch_cap,
+                     (b->flags & QCS_HIP_BATCH_KEEP_STATES) ? 1 : 0);
+  return launch_check("k_batch");
+}
+
+template <int NTH>
+static int occupancy(int lds_bytes, int *per_cu) {
+  const void *fn = reinterpret_cast<const void *>(&k_batch<NTH>);
+  QCS_HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  lds_bytes));
+  QCS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, fn, NTH,
+                                                           size_t(lds_bytes)));
+  return 0;
+}
+
+static int batch_occupancy(int nth, int lds_bytes, int *per_cu) {
+  switch (nth) {
+    case 256: return occupancy<256>(lds_bytes, per_cu);
+    default: return occupancy<512>(lds_bytes, per_cu);
+  }
+}
+
+static int batch_launch(qcs_hip_batch *b, int nth, const BatchLayout &L, int nc,
+                        uint64_t shots, int tile_bytes, int nch_cap, int grid) {
+  switch (nth) {
+    case 256: return launch_batch<256>(b, L, nc, shots, tile_bytes, nch_cap, grid);
+    default: return launch_batch<512>(b, L, nc, shots, tile_bytes, nch_cap, grid);
+  }
+}
+
+// The launch sequence of one run on b->stream: uploads, composition, queue
+// reset, the persistent kernel, the download of the sampled indices.
+static int enqueue_run(qcs_hip_batch *b, const BatchLayout &L, int nc, int nb,
+                       uint64_t shots, int tile_bytes, int nch_cap, int grid,
+                       bool timed) {
+  char *d = static_cast<char *>(b->d_desc);
+  QCS_HIP_TRY(hipMemcpyAsync(d, b->h_stage, L.total, hipMemcpyHostToDevice,
+                             b->stream));
+  int rc;
+  if (nb > 0 &&
+      (rc = launch_compose(b->stream,
+                           reinterpret_cast<const qcs_hip_gate *>(d + L.hgate),
+                           reinterpret_cast<const qcs_hip_block *>(d + L.hblock),
+                           nb, b->d_w, nullptr)))
+    return rc;
+  hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, b->stream, b->d_queue);
+  if ((rc = launch_check("k_batch_reset"))) return rc;
+  if (timed) QCS_HIP_TRY(hipEventRecord(b->ev[1], b->stream));
+  if ((rc = batch_launch(b, b->nth, L, nc, shots, tile_bytes, nch_cap, grid)))
+    return rc;
+  if (timed) QCS_HIP_TRY(hipEventRecord(b->ev[2], b->stream));
+  const size_t obytes = sizeof(long) * size_t(nc) * size_t(shots);
+  if (obytes)
+    QCS_HIP_TRY(hipMemcpyAsync(b->h_out, b->d_out, obytes, hipMemcpyDeviceToHost,
+                               b->stream));
+  return 0;
+}
+
+int batch_run(qcs_hip_batch *b, const qcs_hip_batch_circuit *cs, int nc,
+              const qcs_hip_pass *passes, int np, const qcs_hip_block *blocks,
+              int nb, const qcs_hip_gate *gates, int ng, long shots,
+              const double *u, long *out) {
+  size_t n_bgate = 0, n_bblock = 0;
+  int rc = validate_batch(cs, nc, passes, np, blocks, nb, gates, ng, shots, u,
+                          out, &n_bgate, &n_bblock);
+  if (rc) return rc;
+  if (nc == 0) return 0;
+  QCS_HIP_TRY(hipSetDevice(b->device));
+
+  // layout of the descriptor block (device and pinned staging alike)
+  BatchLayout L;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = off;
+    off = align_up(off + bytes);
+    return at;
+  };
+  L.circ = take(sizeof(BCirc) * size_t(nc));
+  L.order = take(sizeof(int) * size_t(nc));
+  L.pass = take(sizeof(BPass) * size_t(np));
+  L.gate = take(sizeof(BGate) * (n_bgate ? n_bgate : 1));
+  L.block = take(sizeof(BBlock) * (n_bblock ? n_bblock : 1));
+  L.hgate = take(sizeof(qcs_hip_gate) * size_t(ng > 0 ? ng : 1));
+  L.hblock = take(sizeof(qcs_hip_block) * size_t(nb > 0 ? nb : 1));
+  L.u = take(sizeof(double) * size_t(nc) * size_t(shots));
+  L.total = off;
+
+  // state slots and per-circuit cost
+  const bool keep = (b->flags & QCS_HIP_BATCH_KEEP_STATES) != 0;
+  std::vector<uint64_t> slot(size_t(nc), kNoSlot);
+  std::vector<double> cost(size_t(nc), 0.0);
+  size_t states_bytes = 0;
+  int max_tile_bytes = 0, max_n = 1;
+  for (int i = 0; i < nc; ++i) {
+    const qcs_hip_batch_circuit &c = cs[i];
+    const size_t amp = c.precision == QCS_HIP_C128 ? 16 : 8;
+    const bool resident = c.tile_bits == c.num_qubits;
+    if (!resident || keep) {
+      slot[i] = states_bytes;
+      states_bytes = align_up(states_bytes + (amp << c.num_qubits));
+    }
+    max_tile_bytes = std::max(max_tile_bytes, int(amp << c.tile_bits));
+    max_n = std::max(max_n, c.num_qubits);
+    double ops = 0.0;
+    for (int p = c.first_pass; p < c.first_pass + c.num_passes; ++p) {
+      if (c.precision == QCS_HIP_C128) {
+        ops += passes[p].num_blocks + (resident ? 0.0 : 4.0);
+      } else {
+        for (int bi = passes[p].first_block;
+             bi < passes[p].first_block + passes[p].num_blocks; ++bi)
+          ops += 0.25 * double(1 << blocks[bi].k);
+        ops += resident ? 0.0 : 2.0;
+      }
+    }
+    cost[i] = double(amp) * double(uint64_t(1) << c.num_qubits) * (ops + 1.0);
+  }
+  const int nch_cap = int(std::max<uint64_t>(
+      1, (uint64_t(1) << max_n) / std::min<uint64_t>(uint64_t(1) << max_n, kChunk)));
+  const int lds_bytes = max_tile_bytes + int(2 * sizeof(double)) * nch_cap;
+
+  // buffers
+  if ((rc = grow_device(b, &b->d_desc, &b->desc_cap, L.total))) return rc;
+  if ((rc = grow_pinned(b, reinterpret_cast<void **>(&b->h_stage), &b->stage_cap,
+                        L.total)))
+    return rc;
+  {
+    void *p = b->d_w;
+    if ((rc = grow_device(b, &p, &b->w_cap, sizeof(float) * kWSlotFloats *
+                                                 size_t(nb > 0 ? nb : 1))))
+      return rc;
+    b->d_w = static_cast<float *>(p);
+    p = b->d_states;
+    if ((rc = grow_device(b, &p, &b->states_cap, states_bytes ? states_bytes : kBAlign)))
+      return rc;
+    b->d_states = static_cast<char *>(p);
+    const size_t obytes = sizeof(long) * std::max<size_t>(1, size_t(nc) * size_t(shots));
+    p = b->d_out;
+    if ((rc = grow_device(b, &p, &b->out_cap, obytes))) return rc;
+    b->d_out = static_cast<long *>(p);
+    p = b->h_out;
+    if ((rc = grow_pinned(b, &p, &b->hout_cap, obytes))) return rc;
+    b->h_out = static_cast<long *>(p);
+  }
+
+  // descriptors into the pinned staging block
+  char *h = b->h_stage;
+  BCirc *hc = reinterpret_cast<BCirc *>(h + L.circ);
+  int *ho = reinterpret_cast<int *>(h + L.order);
+  BPass *hp = reinterpret_cast<BPass *>(h + L.pass);
+  BGate *hg = reinterpret_cast<BGate *>(h + L.gate);
+  BBlock *hb = reinterpret_cast<BBlock *>(h + L.block);
+  size_t ig = 0, ib = 0;
+  std::vector<char> pass_done(size_t(np), 0);
+  for (int i = 0; i < nc; ++i) {
+    const qcs_hip_batch_circuit &c = cs[i];
+    BCirc &d = hc[i];
+    std::memset(&d, 0, sizeof(d));
+    d.n = c.num_qubits;
+    d.prec = c.precision;
+    d.tb = c.tile_bits;
+    d.npass = c.num_passes;
+    d.first_pass = c.first_pass;
+    d.state_off = slot[i];
+    d.shot_off = uint64_t(i) * uint64_t(shots);
+    for (int p = c.first_pass; p < c.first_pass + c.num_passes; ++p) {
+      // a pass shared by several circuits is written once; its ops then
+      // follow the first circuit's precision, which validation has checked
+      // against each of them
+      if (pass_done[size_t(p)]) continue;
+      pass_done[size_t(p)] = 1;
+      const qcs_hip_pass &ps = passes[p];
+      BPass &dp = hp[p];
+      std::memset(&dp, 0, sizeof(dp));
+      for (int j = 0; j < ps.tile_bits; ++j) dp.sbits[j] = ps.bits[j];
+      dp.nops = ps.num_blocks;
+      if (c.precision == QCS_HIP_C128) {
+        dp.first_op = int(ig);
+        for (int g = ps.first_block; g < ps.first_block + ps.num_blocks; ++g)
+          hg[ig++] = make_bgate(ps, gates[g]);
+      } else {
+        dp.first_op = int(ib);
+        for (int bi = ps.first_block; bi < ps.first_block + ps.num_blocks; ++bi)
+          hb[ib++] = make_bblock(ps, blocks[bi], bi, ps.tile_bits >= 5);
+      }
+    }
+  }
+  // a pass used by circuits of both precisions cannot be encoded twice
+  for (int i = 0; i < nc; ++i)
+    for (int p = cs[i].first_pass; p < cs[i].first_pass + cs[i].num_passes; ++p)
+      for (int j = 0; j < i; ++j)
+        if (p >= cs[j].first_pass && p < cs[j].first_pass + cs[j].num_passes &&
+            cs[j].precision != cs[i].precision)
+          return fail(QCS_HIP_EINVAL, "pass %d shared by circuits of both precisions", p);
+  if (ng > 0) std::memcpy(h + L.hgate, gates, sizeof(qcs_hip_gate) * size_t(ng));
+  if (nb > 0) std::memcpy(h + L.hblock, blocks, sizeof(qcs_hip_block) * size_t(nb));
+  if (shots > 0)
+    std::memcpy(h + L.u, u, sizeof(double) * size_t(nc) * size_t(shots));
+  {
+    std::vector<int> idx(static_cast<size_t>(nc));
+    for (int i = 0; i < nc; ++i) idx[size_t(i)] = i;
+    std::stable_sort(idx.begin(), idx.end(),
+                     [&](int a, int c) { return cost[size_t(a)] > cost[size_t(c)]; });
+    std::memcpy(ho, idx.data(), sizeof(int) * size_t(nc));
+  }
+
+  // persistent grid: every work-group resident at once
+  int per_cu = 0;
+  if ((rc = batch_occupancy(b->nth, lds_bytes, &per_cu))) return rc;
+  if (per_cu < 1) return fail(QCS_HIP_EDEVICE, "batch kernel does not fit on a CU");
+  const int grid = std::min(nc, per_cu * b->num_cus);
+
+  const bool use_graph = (b->flags & QCS_HIP_BATCH_GRAPH) != 0;
+  std::vector<long> key = {b->generation, nc, np, nb, ng, long(n_bgate),
+                           long(n_bblock), shots, b->nth, lds_bytes,
+                           grid, long(L.total), long(states_bytes)};
+  QCS_HIP_TRY(hipEventRecord(b->ev[0], b->stream));
+  b->replayed = 0;
+  if (use_graph) {
+    if (!b->exec || b->graph_key != key) {
+      drop_graph(b);
+      QCS_HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      rc = enqueue_run(b, L, nc, nb, uint64_t(shots), max_tile_bytes, nch_cap,
+                       grid, false);
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(b->stream, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (e != hipSuccess) return check_hip(e, "hipStreamEndCapture");
+      b->graph = g;
+      QCS_HIP_TRY(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+      b->graph_key = key;
+    } else {
+      b->replayed = 1;
+    }
+    QCS_HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+  } else {
+    if ((rc = enqueue_run(b, L, nc, nb, uint64_t(shots), max_tile_bytes, nch_cap,
+                          grid, true)))
+      return rc;
+  }
+  QCS_HIP_TRY(hipEventRecord(b->ev[3], b->stream));
+  QCS_HIP_TRY(hipStreamSynchronize(b->stream));
+  float ms = 0.0f;
+  QCS_HIP_TRY(hipEventElapsedTime(&ms, b->ev[0], b->ev[3]));
+  b->total_ms = ms;
+  b->kernel_ms = -1.0f;
+  if (!use_graph) {
+    QCS_HIP_TRY(hipEventElapsedTime(&ms, b->ev[1], b->ev[2]));
+    b->kernel_ms = ms;
+  }
+  if (shots > 0)
+    std::memcpy(out, b->h_out, sizeof(long) * size_t(nc) * size_t(shots));
+  b->last_n.assign(size_t(nc), 0);
+  b->last_prec.assign(size_t(nc), 0);
+  b->last_off = slot;
+  for (int i = 0; i < nc; ++i) {
+    b->last_n[size_t(i)] = cs[i].num_qubits;
+    b->last_prec[size_t(i)] = cs[i].precision;
+  }
+  return 0;
+}
+
+}  // namespace qcs
+
+using namespace qcs;
+
+extern "C" {
+
+int qcs_hip_batch_create(qcs_hip_batch **out, int flags) {
+  if (!out) return fail(QCS_HIP_EINVAL, "null output handle");
+  *out = nullptr;
+  if (flags & ~(QCS_HIP_BATCH_KEEP_STATES | QCS_HIP_BATCH_GRAPH))
+    return fail(QCS_HIP_EINVAL, "unknown flags %d", flags);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+    return fail(QCS_HIP_ENODEV, "no HIP device available");
+  qcs_hip_batch *b = new (std::nothrow) qcs_hip_batch();
+  if (!b) return fail(QCS_HIP_ENOMEM, "host allocation failed");
+  b->flags = flags;
+  int rc = 0;
+  hipError_t e;
+  if ((e = hipGetDevice(&b->device)) != hipSuccess) rc = check_hip(e, "hipGetDevice");
+  if (!rc && (e = hipDeviceGetAttribute(&b->num_cus, hipDeviceAttributeMultiprocessorCount,
+                                        b->device)) != hipSuccess)
+    rc = check_hip(e, "hipDeviceGetAttribute");
+  if (!rc && (e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess)
+    rc = check_hip(e, "hipStreamCreate");
+  for (int i = 0; i < 4 && !rc; ++i)
+    if ((e = hipEventCreate(&b->ev[i])) != hipSuccess) rc = check_hip(e, "hipEventCreate");
+  if (!rc && (e = hipMalloc(reinterpret_cast<void **>(&b->d_queue), kBAlign)) != hipSuccess)
+    rc = check_hip(e, "hipMalloc(queue)");
+  if (rc) {
+    qcs_hip_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return 0;
+}
+
+void qcs_hip_batch_destroy(qcs_hip_batch *b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  drop_graph(b);
+  if (b->d_desc) (void)hipFree(b->d_desc);
+  if (b->d_w) (void)hipFree(b->d_w);
+  if (b->d_states) (void)hipFree(b->d_states);
+  if (b->d_out) (void)hipFree(b->d_out);
+  if (b->d_queue) (void)hipFree(b->d_queue);
+  if (b->h_stage) (void)hipHostFree(b->h_stage);
+  if (b->h_out) (void)hipHostFree(b->h_out);
+  for (int i = 0; i < 4; ++i)
+    if (b->ev[i]) (void)hipEventDestroy(b->ev[i]);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+int qcs_hip_batch_set_tuning(qcs_hip_batch *b, int knob, int value) {
+  if (!b) return fail(QCS_HIP_EINVAL, "null batch");
+  if (knob == QCS_HIP_TUNE_BATCH_THREADS) {
+    if (value != 256 && value != 512)
+      return fail(QCS_HIP_EINVAL, "batch threads %d: use 256 or 512", value);
+    if (value != b->nth) drop_graph(b);
+    b->nth = value;
+    return 0;
+  }
+  return fail(QCS_HIP_EINVAL, "unknown batch tuning knob %d", knob);
+}
+
+int qcs_hip_batch_run(qcs_hip_batch *b, const qcs_hip_batch_circuit *circuits,
+                      int num_circuits, const qcs_hip_pass *passes,
+                      int num_passes, const qcs_hip_block *blocks,
+                      int num_blocks, const qcs_hip_gate *gates, int num_gates,
+                      long shots, const double *u, long *index_out) {
+  if (!b) return fail(QCS_HIP_EINVAL, "null batch");
+  return batch_run(b, circuits, num_circuits, passes, num_passes, blocks,
+                   num_blocks, gates, num_gates, shots, u, index_out);
+}
+
+int qcs_hip_batch_read_state(qcs_hip_batch *b, int circuit, long offset,
+                             long count, double *out) {
+  if (!b) return fail(QCS_HIP_EINVAL, "null batch");
+  if (!(b->flags & QCS_HIP_BATCH_KEEP_STATES))
+    return fail(QCS_HIP_EUNSUP, "batch was created without QCS_HIP_BATCH_KEEP_STATES");
+  if (circuit < 0 || size_t(circuit) >= b->last_n.size())
+    return fail(QCS_HIP_EINVAL, "circuit %d not in the last run", circuit);
+  const int n = b->last_n[size_t(circuit)];
+  const long N = long(1) << n;
+  if (offset < 0 || count < 0 || offset > N || count > N - offset)
+    return fail(QCS_HIP_EINVAL, "range [%ld, %ld+%ld) outside 2^%d amplitudes",
+                offset, offset, count, n);
+  if (count > 0 && !out) return fail(QCS_HIP_EINVAL, "null buffer");
+  if (count == 0) return 0;
+  QCS_HIP_TRY(hipSetDevice(b->device));
+  const char *src = b->d_states + b->last_off[size_t(circuit)];
+  if (b->last_prec[size_t(circuit)] == QCS_HIP_C128) {
+    QCS_HIP_TRY(hipMemcpy(out, src + 16 * size_t(offset), 16 * size_t(count),
+                          hipMemcpyDeviceToHost));
+    return 0;
+  }
+  std::vector<float> tmp(2 * size_t(count));
+  QCS_HIP_TRY(hipMemcpy(tmp.data(), src + 8 * size_t(offset), 8 * size_t(count),
+                        hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < tmp.size(); ++i) out[i] = double(tmp[i]);
+  return 0;
+}
+
+int qcs_hip_batch_last_timing(qcs_hip_batch *b, double *kernel_ms,
+                              double *total_ms, int *graph_replayed) {
+  if (!b) return fail(QCS_HIP_EINVAL, "null batch");
+  if (kernel_ms) *kernel_ms = b->kernel_ms;
+  if (total_ms) *total_ms = b->total_ms;
+  if (graph_replayed) *graph_replayed = b->replayed;
+  return 0;
+}
+
+}  // extern "C"

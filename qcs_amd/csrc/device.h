@@ -96,6 +96,16 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// Inclusive prefix sum over the 64 lanes of a wave (Hillis-Steele order).
+__device__ __forceinline__ double wave_incl_scan(double x, int lane) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const double y = __shfl_up(x, d, 64);
+    if (lane >= d) x += y;
+  }
+  return x;
+}
+
 // Sum of v over the work-group; `lds` holds blockDim.x/64 doubles.
 // The result is valid in thread 0.
 __device__ __forceinline__ double block_sum(double v, double *lds) {

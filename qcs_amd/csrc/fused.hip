@@ -87,6 +87,7 @@ __global__ __launch_bounds__(64) void k_compose(const qcs_hip_gate *gates,
     const double v = (p == q) ? ur : (p == 0 ? -ui : ui);
     W[idx] = float(v);
   }
+  if (!ubuf) return;
   double *U = ubuf + size_t(blockIdx.x) * kUDoubles;
   for (int idx = threadIdx.x; idx < R * R; idx += 64) {
     const int i = idx / R, jj = idx % R;
@@ -240,6 +241,15 @@ int compose_blocks(qcs_hip_state *s, const qcs_hip_block *blocks, int nb,
                      s->ubuf);
   if ((rc = launch_check("k_compose"))) return rc;
   return timing_end(s, &tl);
+}
+
+int launch_compose(hipStream_t stream, const qcs_hip_gate *gates,
+                   const qcs_hip_block *blocks, int nb, float *wbuf,
+                   double *ubuf) {
+  if (nb <= 0) return 0;
+  hipLaunchKernelGGL(k_compose, dim3(nb), dim3(64), 0, stream, gates, blocks,
+                     wbuf, ubuf);
+  return launch_check("k_compose");
 }
 
 int fused_apply(qcs_hip_state *s, const qcs_hip_block *blocks, int nb,

@@ -200,13 +200,13 @@ static Mat2<T> to_mat(const double m[8]) {
   return r;
 }
 
-static bool is_phase1(const double m[8]) {
+bool is_phase1(const double m[8]) {
   // diag(1, x): m00 = 1, m01 = m10 = 0
   return m[0] == 1.0 && m[1] == 0.0 && m[2] == 0.0 && m[3] == 0.0 &&
          m[4] == 0.0 && m[5] == 0.0;
 }
 
-static bool is_x(const double m[8]) {
+bool is_x(const double m[8]) {
   return m[0] == 0.0 && m[1] == 0.0 && m[2] == 1.0 && m[3] == 0.0 &&
          m[4] == 1.0 && m[5] == 0.0 && m[6] == 0.0 && m[7] == 0.0;
 }

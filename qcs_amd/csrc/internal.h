@@ -93,6 +93,11 @@ int check_hip(hipError_t e, const char *what);
     if (qcs_e_ != hipSuccess) return qcs::check_hip(qcs_e_, #expr); \
   } while (0)
 
+// fused blocks of an n-qubit state: k, ascending targets, gate ranges and
+// gates inside their block (0 or QCS_HIP_EINVAL with the message set)
+int validate_blocks(int n, const qcs_hip_block *blocks, int num_blocks,
+                    const qcs_hip_gate *gates, int num_gates);
+
 // launch bracketing for the kernel timers
 void timing_begin(qcs_hip_state *s, int cls, double bytes, double flops,
                   TimedLaunch *tl);
@@ -108,6 +113,10 @@ int launch_1q(qcs_hip_state *s, int target, const double m[8]);
 int launch_cu(qcs_hip_state *s, int control, int target, const double m[8]);
 int launch_swap(qcs_hip_state *s, int a, int b);
 int launch_fill_basis(qcs_hip_state *s, void *buf, uint64_t basis, int one);
+// gate classes of the plain kernels: diag(1, x) (only the bit-1 half
+// changes) and X (a pure exchange)
+bool is_phase1(const double m[8]);
+bool is_x(const double m[8]);
 
 // kernels in reduce.hip
 int reduce_norm(qcs_hip_state *s, double *out_host);  // host result
@@ -131,6 +140,12 @@ int fused_apply_unitary(qcs_hip_state *s, int k, const int *targets,
 int compose_blocks(qcs_hip_state *s, const qcs_hip_block *blocks, int nb,
                    const qcs_hip_gate *gates, int ng, size_t extra,
                    char **extra_stage);
+// k_compose over nb device blocks: W slots of 64 * 64 floats per block, U
+// slots (skipped when ubuf is null) of 32 * 32 complex doubles
+constexpr int kWSlotFloats = 64 * 64;
+int launch_compose(hipStream_t stream, const qcs_hip_gate *gates,
+                   const qcs_hip_block *blocks, int nb, float *wbuf,
+                   double *ubuf);
 
 // kernels in tiles.hip
 int tile_max_blocks();

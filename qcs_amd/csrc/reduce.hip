@@ -224,14 +224,6 @@ __global__ __launch_bounds__(kThreads) void k_argmax_final(
 // The sampling kernel recomputes local(i) with the same operations, so
 // cdf(last of chunk c) == P[c] bit for bit.
 
-__device__ __forceinline__ double wave_incl_scan(double x, int lane) {
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const double y = __shfl_up(x, d, 64);
-    if (lane >= d) x += y;
-  }
-  return x;
-}
 
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_chunk_tot(

@@ -373,3 +373,78 @@ out:
   *ordered_out = ordered;
   return 0;
 }
+
+/* close the pass in `ps`: fill S up to tile_bits with the lowest qubits not
+ * in it, write the bits ascending and clear `in` */
+static void close_gate_pass(qcs_hip_pass *ps, char *in, int n, int tile_bits,
+                            int size) {
+  int q, k;
+  for (q = 0; q < n && size < tile_bits; q++)
+    if (!in[q]) {
+      in[q] = 1;
+      size++;
+    }
+  ps->tile_bits = tile_bits;
+  for (q = 0, k = 0; q < n; q++)
+    if (in[q]) {
+      if (k < tile_bits) ps->bits[k++] = q;
+      in[q] = 0;
+    }
+}
+
+int qcs_gate_pass_plan(const qcs_hip_gate *gates, int ng, int n, int tile_bits,
+                       int low_bits, qcs_hip_pass **passes_out,
+                       int *npasses_out) {
+  qcs_hip_pass *passes;
+  char *in;
+  int np = 0, size = 0, i, q;
+
+  *passes_out = NULL;
+  *npasses_out = 0;
+  if (ng < 0 || n < 1 || tile_bits < 1 || tile_bits > 16 || low_bits < 0)
+    return -1;
+  if (tile_bits > n) tile_bits = n;
+  if (low_bits > tile_bits) low_bits = tile_bits;
+  /* leave room for a 2-qubit gate outside the low qubits */
+  if (tile_bits < n && low_bits > tile_bits - 2)
+    low_bits = tile_bits >= 2 ? tile_bits - 2 : 0;
+  for (i = 0; i < ng; i++) {
+    int qs[2], nq = gate_qubits(&gates[i], qs), j;
+    for (j = 0; j < nq; j++)
+      if (qs[j] < 0 || qs[j] >= n) return -1;
+    if (nq == 2 && qs[0] == qs[1]) return -1;
+  }
+  passes = (qcs_hip_pass *)calloc((size_t)ng + 1, sizeof(qcs_hip_pass));
+  in = (char *)calloc((size_t)n, 1);
+  if (!passes || !in) {
+    free(passes);
+    free(in);
+    return -1;
+  }
+  for (q = 0; q < low_bits; q++) in[q] = 1;
+  size = low_bits;
+  passes[0].first_block = 0;
+  for (i = 0; i < ng; i++) {
+    int qs[2], nq = gate_qubits(&gates[i], qs), add = 0, j;
+    for (j = 0; j < nq; j++) add += !in[qs[j]];
+    if (size + add > tile_bits) {
+      close_gate_pass(&passes[np], in, n, tile_bits, size);
+      np++;
+      passes[np].first_block = i;
+      for (q = 0; q < low_bits; q++) in[q] = 1;
+      size = low_bits;
+    }
+    for (j = 0; j < nq; j++)
+      if (!in[qs[j]]) {
+        in[qs[j]] = 1;
+        size++;
+      }
+    passes[np].num_blocks++;
+  }
+  close_gate_pass(&passes[np], in, n, tile_bits, size);
+  np++;
+  free(in);
+  *passes_out = passes;
+  *npasses_out = np;
+  return 0;
+}

@@ -32,6 +32,17 @@ static int is_compat(const t_q_circuit *c) {
   return (c->options & QCS_REFERENCE_COMPAT) != 0;
 }
 
+/* deferred circuits hold no device state: gates are only recorded, for a
+ * batched run (batch.c) */
+static int is_deferred(const t_q_circuit *c) {
+  return (c->options & QCS_DEFERRED) != 0;
+}
+
+/* gate calls may proceed: a device state, or a deferred recording */
+static int can_apply(const t_q_circuit *c) {
+  return c != NULL && (c->state != NULL || is_deferred(c));
+}
+
 /* record a backend failure: message on stderr, sticky error code */
 static int check(t_q_circuit *c, int rc, const char *what) {
   if (rc != 0) {
@@ -195,11 +206,11 @@ static void flush_pending(t_q_circuit *c) {
 /* ---------------------------------------------------------------------- */
 
 static void apply_1q(t_q_circuit *c, int q, const double m[8]) {
-  if (c == NULL || c->state == NULL || q < 0 || q >= c->num_qubits) {
+  if (!can_apply(c) || q < 0 || q >= c->num_qubits) {
     fprintf(stderr, "Error: Invalid arguments for 1-qubit gate application.\n");
     return;
   }
-  if (is_fused(c)) {
+  if (is_fused(c) || is_deferred(c)) {
     record(c, QCS_HIP_G_1Q, q, -1, m);
     return;
   }
@@ -208,13 +219,13 @@ static void apply_1q(t_q_circuit *c, int q, const double m[8]) {
 
 static void apply_2q(t_q_circuit *c, int control, int target,
                      const double m[8]) {
-  if (c == NULL || c->state == NULL || control < 0 || target < 0 ||
+  if (!can_apply(c) || control < 0 || target < 0 ||
       control >= c->num_qubits || target >= c->num_qubits ||
       control == target) {
     fprintf(stderr, "Error: Invalid arguments for 2-qubit gate application.\n");
     return;
   }
-  if (is_fused(c)) {
+  if (is_fused(c) || is_deferred(c)) {
     record(c, QCS_HIP_G_CU, target, control, m);
     return;
   }
@@ -251,6 +262,16 @@ t_q_circuit *qc_create_ex(int num_qubits, int options) {
     /* q_state_init's message; the circuit exists without a state
        (src/q_state.c:38-41, src/qcs.c:54) */
     fprintf(stderr, "Error: Number of qubits must be positive.\n");
+    return c;
+  }
+  if (options & QCS_DEFERRED) {
+    if ((options & (QCS_PLAIN_FP32 | QCS_REFERENCE_COMPAT | QCS_FUSE_TILES)) ||
+        num_qubits > QCS_HIP_BATCH_MAX_QUBITS) {
+      fprintf(stderr, "Error: QCS_DEFERRED takes 1..%d qubits, complex128 "
+                      "plain or QCS_FUSED_FP32.\n", QCS_HIP_BATCH_MAX_QUBITS);
+      qc_destroy(c);
+      return NULL;
+    }
     return c;
   }
   precision = (options & (QCS_FUSED_FP32 | QCS_PLAIN_FP32)) ? QCS_HIP_C64
@@ -364,12 +385,12 @@ void qc_cphase(t_q_circuit *circuit, int control, int target, double angle) {
 }
 
 void qc_swap(t_q_circuit *circuit, int qubit_a, int qubit_b) {
-  if (circuit == NULL || circuit->state == NULL || qubit_a < 0 ||
+  if (!can_apply(circuit) || qubit_a < 0 ||
       qubit_b < 0 || qubit_a >= circuit->num_qubits ||
       qubit_b >= circuit->num_qubits || qubit_a == qubit_b ||
       is_compat(circuit)) {
     fprintf(stderr, "Error: Invalid arguments for swap.\n");
-  } else if (is_fused(circuit)) {
+  } else if (is_fused(circuit) || is_deferred(circuit)) {
     record(circuit, QCS_HIP_G_SWAP, qubit_a, qubit_b, NULL);
   } else {
     check(circuit, qcs_hip_apply_swap(circuit->state, qubit_a, qubit_b), "swap");

@@ -45,4 +45,15 @@ int qcs_pass_plan(const qcs_hip_gate *gates, int ng, int n, int tile_bits,
                   int *npasses, qcs_hip_block **blocks, int *nblocks,
                   qcs_hip_gate **ordered);
 
+/* Pass planner for gate-by-gate application (batched complex128 circuits):
+ * the gates stay in circuit order; each pass is a maximal run of
+ * consecutive gates whose qubits, together with qubits 0..low_bits-1, fit in
+ * `tile_bits` qubits (filled up with the lowest unused qubits; low_bits is
+ * cut to tile_bits - 2 when the state is larger than a tile, so every gate
+ * fits a pass of its own).  A pass's
+ * first_block / num_blocks index gates.  With n <= tile_bits there is one
+ * pass of n bits (also when ng == 0). */
+int qcs_gate_pass_plan(const qcs_hip_gate *gates, int ng, int n, int tile_bits,
+                       int low_bits, qcs_hip_pass **passes, int *npasses);
+
 #endif

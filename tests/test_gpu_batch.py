@@ -1,0 +1,184 @@
+"""Batched circuits on the GPU (qc_batch_*, one persistent launch).
+
+complex128 circuits must equal a circuit of their own (qc_create + the same
+gates + qc_sample) bit for bit: every amplitude and every sampled index from
+the same rand() stream.  Fused complex64 circuits must stay within 1 - F <=
+1e-6 of the complex128 result and sample the same indices wherever the FP32
+error cannot move a CDF boundary across a uniform.  Sizes cover states that
+stay in LDS (n <= tile bits), states streamed through LDS tiles, the
+boundary between them, one-qubit and gate-free circuits."""
+import numpy as np
+import pytest
+
+import qcs_amd
+from qcs_amd import circuits as C
+
+pytestmark = pytest.mark.gpu
+
+ALL = [C.H, C.X, C.Y, C.Z, C.P, C.RX, C.RY, C.RZ, C.CNOT, C.CPHASE, C.SWAP]
+
+
+def make_ops(n, seed, count=None):
+    if count is None:
+        return C.layered(n, 3, seed) if seed % 2 else C.random_circuit(
+            n, 12 * n, seed, kinds=ALL)
+    return C.random_circuit(n, count, seed, kinds=ALL)
+
+
+def batch_run(specs, options, shots, cl, seed, batch_opts=qcs_amd.BATCH_KEEP_STATES,
+              plans=(), threads=None):
+    circs = []
+    for n, ops in specs:
+        c = qcs_amd.Circuit(n, qcs_amd.DEFERRED | options)
+        c.apply(ops)
+        circs.append(c)
+    b = qcs_amd.Batch(batch_opts)
+    for p in plans:
+        b.set_plan(*p)
+    if threads:
+        b.set_threads(threads)
+    cl.srand(seed)
+    idx = b.sample(circs, shots)
+    amps = [b.amplitudes(i, n) for i, (n, _) in enumerate(specs)]
+    for c in circs:
+        c.close()
+    return b, amps, idx
+
+
+SIZES = [1, 2, 3, 5, 9, 12, 13, 14, 15, 16]
+
+
+def test_c128_batch_bit_identical(qlib, cl):
+    specs = [(n, make_ops(n, 100 + n)) for n in SIZES]
+    b, amps, idx = batch_run(specs, qcs_amd.PLAIN_FP64, 200, cl, 7)
+    # the single runs draw the uniforms circuit after circuit, like the batch
+    cl.srand(7)
+    for i, (n, ops) in enumerate(specs):
+        c = qcs_amd.Circuit(n)
+        c.apply(ops)
+        ref_i = c.sample_indices(200)
+        ref_a = c.amplitudes()
+        c.close()
+        assert np.array_equal(amps[i], ref_a), f"n={n}: amplitudes differ"
+        assert np.array_equal(idx[i], ref_i), f"n={n}: samples differ"
+    b.close()
+
+
+@pytest.mark.parametrize("tb", [6, 8, 11])
+def test_c128_streamed_tiles_match_resident(qlib, cl, tb):
+    # states larger than the tile go through HBM pass by pass
+    specs = [(n, make_ops(n, 300 + n)) for n in (tb + 1, tb + 3, 14)]
+    plans = [(0, n, tb, 0) for n, _ in specs]
+    _, amps, idx = batch_run(specs, qcs_amd.PLAIN_FP64, 64, cl, 3, plans=plans)
+    _, amps0, idx0 = batch_run(specs, qcs_amd.PLAIN_FP64, 64, cl, 3)
+    for i in range(len(specs)):
+        assert np.array_equal(amps[i], amps0[i])
+        assert np.array_equal(idx[i], idx0[i])
+
+
+def fidelity(a, b):
+    return 1.0 - abs(np.vdot(a, b)) ** 2 / (np.vdot(a, a).real * np.vdot(b, b).real)
+
+
+def test_c64_fused_batch_within_tolerance(qlib, cl):
+    specs = [(n, make_ops(n, 500 + n)) for n in SIZES + [17]]
+    _, amps, idx = batch_run(specs, qcs_amd.FUSED_FP32, 256, cl, 11)
+    _, amps64, idx64 = batch_run(specs, qcs_amd.PLAIN_FP64, 256, cl, 11)
+    for i, (n, _) in enumerate(specs):
+        omf = fidelity(amps[i], amps64[i])
+        assert omf <= 1e-6, (n, omf)
+        assert np.max(np.abs(amps[i] - amps64[i])) <= 1e-5, n
+        assert abs(np.vdot(amps[i], amps[i]).real - 1.0) <= 1e-6, n
+        assert np.mean(idx[i] == idx64[i]) >= 0.99, n
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5])
+def test_c64_fusion_widths_and_streaming(qlib, cl, k):
+    specs = [(n, make_ops(n, 700 + n + k)) for n in (6, 11, 15)]
+    plans = [(1, n, 0, k) for n, _ in specs] + [(1, 15, 9, k)]
+    _, amps, _ = batch_run(specs, qcs_amd.FUSED_FP32, 16, cl, 5, plans=plans)
+    _, amps64, _ = batch_run(specs, qcs_amd.PLAIN_FP64, 16, cl, 5)
+    for i, (n, _) in enumerate(specs):
+        assert fidelity(amps[i], amps64[i]) <= 1e-6, (n, k)
+
+
+def test_mixed_batch_matches_separate_batches(qlib, cl):
+    specs = [(n, make_ops(n, 900 + n)) for n in (4, 10, 14, 16)]
+    circs = []
+    for j, (n, ops) in enumerate(specs * 2):
+        opt = qcs_amd.FUSED_FP32 if j >= len(specs) else qcs_amd.PLAIN_FP64
+        c = qcs_amd.Circuit(n, qcs_amd.DEFERRED | opt)
+        c.apply(ops)
+        circs.append(c)
+    b = qcs_amd.Batch(qcs_amd.BATCH_KEEP_STATES)
+    cl.srand(21)
+    idx = b.sample(circs, 32)
+    _, a128, i128 = batch_run(specs, qcs_amd.PLAIN_FP64, 32, cl, 21)
+    for i, (n, _) in enumerate(specs):
+        assert np.array_equal(b.amplitudes(i, n), a128[i])
+        assert np.array_equal(idx[i], i128[i])
+        assert fidelity(b.amplitudes(len(specs) + i, n), a128[i]) <= 1e-6
+    for c in circs:
+        c.close()
+    b.close()
+
+
+def test_threads_and_graph_replay(qlib, cl):
+    specs = [(n, make_ops(n, 40 + n)) for n in (3, 8, 13, 15)]
+    _, a512, i512 = batch_run(specs, qcs_amd.PLAIN_FP64, 48, cl, 9)
+    _, a256, i256 = batch_run(specs, qcs_amd.PLAIN_FP64, 48, cl, 9, threads=256)
+    for i in range(len(specs)):
+        assert np.array_equal(a512[i], a256[i]) and np.array_equal(i512[i], i256[i])
+    # graph: first run captures, later runs with the same shape replay; new
+    # angles (same structure) must give the new results
+    b = qcs_amd.Batch(qcs_amd.BATCH_GRAPH | qcs_amd.BATCH_KEEP_STATES)
+    for rep, seed in enumerate((9, 9, 33)):
+        circs = []
+        for n, _ in specs:
+            c = qcs_amd.Circuit(n, qcs_amd.DEFERRED)
+            c.apply(C.layered(n, 2, seed + n))
+            circs.append(c)
+        cl.srand(seed)
+        idx = b.sample(circs, 48)
+        _, _, replayed = b.last_timing()
+        assert replayed == (rep > 0)
+        ref_specs = [(n, C.layered(n, 2, seed + n)) for n, _ in specs]
+        _, ra, ri = batch_run(ref_specs, qcs_amd.PLAIN_FP64, 48, cl, seed)
+        for i, (n, _) in enumerate(specs):
+            assert np.array_equal(b.amplitudes(i, n), ra[i])
+            assert np.array_equal(idx[i], ri[i])
+        for c in circs:
+            c.close()
+    b.close()
+
+
+def test_edges(qlib, cl):
+    # gate-free circuits sample |0...0>; zero shots; run_shots histograms
+    specs = [(n, C.Ops()) for n in (1, 7, 15)]
+    _, amps, idx = batch_run(specs, qcs_amd.PLAIN_FP64, 10, cl, 1)
+    for i, (n, _) in enumerate(specs):
+        assert amps[i][0] == 1 and np.count_nonzero(amps[i]) == 1
+        assert np.all(idx[i] == 0)
+    _, _, idx = batch_run(specs, qcs_amd.FUSED_FP32, 0, cl, 1)
+    assert idx.shape == (3, 0)
+    circs = []
+    for n in (2, 9):
+        c = qcs_amd.Circuit(n, qcs_amd.DEFERRED)
+        c.h(0)
+        circs.append(c)
+    b = qcs_amd.Batch()
+    cl.srand(4)
+    hists = b.run_shots(circs, 1000)
+    for h, c in zip(hists, circs):
+        assert h.sum() == 1000 and 400 <= h[0] <= 600 and h[0] + h[1] == 1000
+        c.close()
+    b.close()
+
+
+def test_rejects_non_deferred(qlib, cl):
+    c = qcs_amd.Circuit(4)
+    b = qcs_amd.Batch()
+    with pytest.raises(RuntimeError):
+        b.sample([c], 4)
+    c.close()
+    b.close()
