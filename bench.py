@@ -54,6 +54,11 @@ def parse():
     p.add_argument("--no-plain", action="store_true", help="skip the plain FP64 run")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
+    p.add_argument("--workload", choices=["fused30", "batch"], default="fused30",
+                   help="fused30: the 30-qubit fused-vs-plain line (default); batch: "
+                        "4096 independent 10-20 qubit circuits in one persistent launch")
+    p.add_argument("--batch-count", type=int, default=4096)
+    p.add_argument("--batch-threads", type=int, default=512)
     p.add_argument("--copy-probe", type=float, default=0.0, metavar="GIB",
                    help="only run a plain device copy of GIB GiB (float32, out = in * 1) "
                         "three times: the known-size kernel that scales HBM byte "
@@ -143,6 +148,104 @@ def _slice(ops, i):
     return o
 
 
+class _Arrays:
+    def __init__(self, a):
+        self.op, self.tq, self.cq, self.par = (a[0].tolist(), a[1].tolist(),
+                                               a[2].tolist(), a[3].tolist())
+
+
+def bench_batch(args):
+    """4096 independent layered circuits (n ~ U[10,20], depth ~ U[5,40],
+    splitmix64 seed 3; even circuits complex128 plain, odd fused complex64),
+    256 shots each, one qc_batch_sample call per step (host planning, one
+    persistent device launch replayed from a hipGraph after the first step,
+    sampled indices back on the host)."""
+    import qcs_amd
+    from qcs_amd import circuits as C
+
+    hip, _ = qcs_amd.load()
+    if not hip.qcs_hip_device_available():
+        raise SystemExit("no HIP device")
+    shots = 256
+    work = C.batch_workload(args.batch_count, 3)
+    t0 = time.perf_counter()
+    circs = []
+    amp_updates = {False: 0.0, True: 0.0}
+    gates = 0
+    for n, d, seed, fused in work:
+        c = qcs_amd.Circuit(n, qcs_amd.DEFERRED | (qcs_amd.FUSED_FP32 if fused else 0))
+        c.apply(_Arrays(C.layered_arrays(n, d, seed)))
+        g = C.gate_count_layered(n, d)
+        gates += g
+        amp_updates[fused] += g * float(1 << n)
+        circs.append(c)
+    t_record = time.perf_counter() - t0
+    print(f"batch: recorded {len(circs)} circuits, {gates} gates in {t_record:.1f} s",
+          file=sys.stderr, flush=True)
+    cl = qcs_amd.libc()
+    cl.srand(3)
+    # one run without a graph times the batch kernel alone (HIP events)
+    b0 = qcs_amd.Batch()
+    b0.set_threads(args.batch_threads)
+    t0 = time.perf_counter()
+    b0.sample(circs, shots)
+    first_s = time.perf_counter() - t0
+    kernel_ms, first_dev_ms, _ = b0.last_timing()
+    b0.close()
+    print(f"batch: first run {first_s:.2f} s, device {first_dev_ms:.1f} ms, "
+          f"kernel {kernel_ms:.1f} ms", file=sys.stderr, flush=True)
+    b = qcs_amd.Batch(qcs_amd.BATCH_GRAPH)
+    b.set_threads(args.batch_threads)
+    for _ in range(max(args.warmup, 1)):
+        b.sample(circs, shots)
+    hip.qcs_hip_device_sync()
+    totals = []
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        b.sample(circs, shots)
+        totals.append(b.last_timing()[1])
+    hip.qcs_hip_device_sync()
+    T = (time.perf_counter() - t1) / args.steps
+    total_amp = amp_updates[False] + amp_updates[True]
+    dev_ms = sum(totals) / len(totals)
+    result = {
+        "metric": "amplitude updates/s, 4096 batched circuits (10-20 qubits), one persistent launch",
+        "value": total_amp / T,
+        "unit": "amplitude updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": max(args.warmup, 1),
+        "ms_per_step": T * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64+f32",
+        "data": "synthetic",
+        "config": {
+            "workload": ("4096 independent layered Rz-Ry-Rz + CNOT-ladder circuits, "
+                         "n ~ U[10,20], depth ~ U[5,40] (splitmix64 seed 3), even "
+                         "circuits complex128 plain, odd fused complex64, 256 shots each"),
+            "circuits": len(circs),
+            "gates": gates,
+            "shots_per_circuit": shots,
+            "threads_per_workgroup": args.batch_threads,
+        },
+        "device_ms_per_step": dev_ms,
+        "host_ms_per_step": T * 1e3 - dev_ms,
+        "kernel_ms": kernel_ms,
+        "device_amp_updates_per_s_kernel": total_amp / (kernel_ms * 1e-3),
+        "device_amp_updates_per_s": total_amp / (dev_ms * 1e-3),
+        "amp_updates_c128": amp_updates[False],
+        "amp_updates_c64": amp_updates[True],
+        "circuits_per_s": len(circs) / T,
+        "record_s": t_record,
+    }
+    for c in circs:
+        c.close()
+    b.close()
+    print(json.dumps(result), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -151,6 +254,9 @@ def main():
         sys.exit(2)
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.workload == "batch":
+        bench_batch(args)
         return
     import qcs_amd
     from qcs_amd import circuits as C

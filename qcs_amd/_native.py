@@ -170,6 +170,8 @@ _QCS_PROTOS = {
                               P(P(HipGate))]),
     "qcs_gate_pass_plan": (c_int, [P(HipGate), c_int, c_int, c_int, c_int,
                                    P(P(HipPass)), P(c_int)]),
+    "qcs_gate_is_exact": (c_int, [P(HipGate)]),
+    "qcs_gate_hoist_exact": (c_int, [P(HipGate), c_int, c_int, P(P(HipGate))]),
     "qc_batch_create": (c_void_p, [c_int]),
     "qc_batch_destroy": (None, [c_void_p]),
     "qc_batch_sample": (c_int, [c_void_p, P(_Q), c_int, c_int, P(c_long)]),

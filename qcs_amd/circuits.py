@@ -117,3 +117,62 @@ def random_circuit(n, count, seed, kinds=None):
 
 def gate_count_layered(n, depth):
     return n + depth * (3 * n + n - 1)
+
+
+def splitmix64_array(seed, count):
+    """The first ``count`` outputs of splitmix64(seed) as a uint64 array."""
+    with np.errstate(over="ignore"):
+        k = np.arange(1, count + 1, dtype=np.uint64)
+        z = np.uint64(seed & _MASK) + k * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def layered_arrays(n, depth, seed, initial_h=True):
+    """``layered(n, depth, seed)`` as four numpy arrays (op, target, control,
+    param), built without a Python loop per gate."""
+    u = (splitmix64_array(seed, 3 * n * depth) >> np.uint64(11)).astype(np.float64)
+    ang = (2.0 * math.pi) * (u * (1.0 / 9007199254740992.0))
+    rot_op = np.tile(np.array([RZ, RY, RZ]), n)
+    rot_q = np.repeat(np.arange(n), 3)
+    cn_c = np.concatenate([np.arange(0, n - 1, 2), np.arange(1, n - 1, 2)])
+    per = 3 * n + len(cn_c)
+    op = np.empty(depth * per, np.int64)
+    tq = np.empty(depth * per, np.int64)
+    cq = np.empty(depth * per, np.int64)
+    par = np.zeros(depth * per, np.float64)
+    for d in range(depth):
+        s = d * per
+        op[s:s + 3 * n] = rot_op
+        tq[s:s + 3 * n] = rot_q
+        cq[s:s + 3 * n] = -1
+        par[s:s + 3 * n] = ang[3 * n * d:3 * n * (d + 1)]
+        op[s + 3 * n:s + per] = CNOT
+        tq[s + 3 * n:s + per] = cn_c + 1
+        cq[s + 3 * n:s + per] = cn_c
+    if initial_h:
+        op = np.concatenate([np.full(n, H), op])
+        tq = np.concatenate([np.arange(n), tq])
+        cq = np.concatenate([np.full(n, -1), cq])
+        par = np.concatenate([np.zeros(n), par])
+    return op, tq, cq, par
+
+
+def batch_workload(count=4096, seed=3, qubits=(10, 20), depths=(5, 40)):
+    """Independent layered circuits for the batched form: circuit i has
+    n = qubits[0] + floor(u * (qubits[1] - qubits[0] + 1)) qubits and depth
+    depths[0] + floor(u' * (depths[1] - depths[0] + 1)), angles from its own
+    seed; u, u' and the seed are consecutive splitmix64(seed) outputs.  Even
+    circuits run complex128 plain, odd ones fused complex64.  Returns a list
+    of (n, depth, circuit_seed, fused)."""
+    z = splitmix64_array(seed, 3 * count).reshape(count, 3)
+    out = []
+    nq = qubits[1] - qubits[0] + 1
+    nd = depths[1] - depths[0] + 1
+    for i in range(count):
+        un = float(int(z[i, 0]) >> 11) * (1.0 / 9007199254740992.0)
+        ud = float(int(z[i, 1]) >> 11) * (1.0 / 9007199254740992.0)
+        out.append((qubits[0] + int(un * nq), depths[0] + int(ud * nd),
+                    int(z[i, 2]), i % 2 == 1))
+    return out

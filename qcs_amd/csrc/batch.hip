@@ -55,8 +55,20 @@ enum : int {
   kOpCuDense = 2, // 2x2 on control=1 pairs
   kOpCnot = 3,    // exchange on control=1 pairs
   kOpCPhase = 4,  // m11 on control=1, target=1
-  kOpSwap = 5     // exchange (1,0) <-> (0,1) of qubits t, c
+  kOpSwap = 5,    // exchange (1,0) <-> (0,1) of qubits t, c
+  // register groups only: the dense 2x2 without the products by exact-zero
+  // entries, which add only signed zeros -- the results compare equal to
+  // kOp1Dense's (they may differ in the sign of a zero)
+  kOp1Diag = 6,   // m01 = m10 = 0 (RZ)
+  kOp1Real = 7,   // every imaginary part 0 (H, RY)
+  kOp1Anti = 8    // m00 = m11 = 0 (X, Y)
 };
+
+// Register groups (complex128): consecutive gates of a pass acting on at
+// most kGR tile bits are applied by one sweep in which every thread holds
+// the kGA amplitudes that differ only in those bits.
+constexpr int kGR = 4;
+constexpr int kGA = 1 << kGR;
 
 struct BCirc {
   int n;
@@ -84,6 +96,13 @@ struct BGate {
   double m[8];
 };
 
+struct BGroup {
+  int first_gate;      // gates with t, c as positions 0..kGR-1 in qpos
+  int ngates;
+  int qpos[kGR];       // tile positions of the group, ascending
+  int rowslot[kGA];    // LDS slot (dswz) of the row pattern j
+};
+
 struct BBlock {
   int k;
   int wslot;
@@ -95,6 +114,14 @@ struct BBlock {
 // and a bijection of every [0, 2^tb), tb >= 5.
 __host__ __device__ __forceinline__ int bswz(int e) {
   return e ^ ((e >> 5) & 31) ^ ((e >> 10) & 15);
+}
+
+// XOR swizzle of a complex128 tile element (double2 slot): bits 4..7 fold
+// onto bits 0..3, so the 16 lanes of a ds_read_b128 quarter-wave that step
+// through a group's columns hit distinct banks for most bit layouts.  Linear
+// over GF(2), a bijection of every [0, 2^tb), the identity below 16.
+__host__ __device__ __forceinline__ int dswz(int e) {
+  return e ^ ((e >> 4) & 15);
 }
 
 __device__ __forceinline__ int bins0(int x, int b) {
@@ -120,6 +147,7 @@ __device__ __forceinline__ uint64_t deposit(uint32_t e, const int *sb, int j0,
 
 // ---- complex128: gates on an LDS tile -------------------------------------------
 
+// (tiles of fewer than kGR bits; there dswz is the identity)
 template <int NTH>
 __device__ void tile_gate(double2 *L, int tb, const BGate *__restrict__ gp) {
   const int cls = gp->cls, t = gp->t, c = gp->c;
@@ -185,29 +213,193 @@ __device__ void tile_gate(double2 *L, int tb, const BGate *__restrict__ gp) {
   }
 }
 
-// complex128 tile <-> HBM: element e <-> G[base | deposit(e)]
+// ---- complex128: register groups ----------------------------------------------
+//
+// v[j] holds the amplitude whose group bits are the bits of j (bit i of j =
+// tile bit qpos[i]); a gate on group positions t (and c) pairs registers at
+// compile-time indices, with the arithmetic of gates.hip.
+
+// The gate class and positions are template parameters, so each variant is
+// straight-line register code; r_gate dispatches once per gate.
+template <int Q, int CLS>
+__device__ __forceinline__ void r_1q(double (&vr)[kGA], double (&vi)[kGA], const double *m) {
+#pragma unroll
+  for (int j = 0; j < kGA; ++j) {
+    if (j & (1 << Q)) continue;
+    const int k = j | (1 << Q);
+    const double2 v0 = make_double2(vr[j], vi[j]), v1 = make_double2(vr[k], vi[k]);
+    double2 o0, o1;
+    if constexpr (CLS == kOp1Dense) {
+      row2(m[0], m[1], m[2], m[3], v0.x, v0.y, v1.x, v1.y, o0.x, o0.y);
+      row2(m[4], m[5], m[6], m[7], v0.x, v0.y, v1.x, v1.y, o1.x, o1.y);
+    } else if constexpr (CLS == kOp1Phase) {
+      o0 = v0;
+      cmul(m[6], m[7], v1.x, v1.y, o1.x, o1.y);
+    } else if constexpr (CLS == kOp1Diag) {
+      cmul(m[0], m[1], v0.x, v0.y, o0.x, o0.y);
+      cmul(m[6], m[7], v1.x, v1.y, o1.x, o1.y);
+    } else if constexpr (CLS == kOp1Real) {
+      o0.x = m[0] * v0.x + m[2] * v1.x;
+      o0.y = m[0] * v0.y + m[2] * v1.y;
+      o1.x = m[4] * v0.x + m[6] * v1.x;
+      o1.y = m[4] * v0.y + m[6] * v1.y;
+    } else {  // kOp1Anti
+      cmul(m[2], m[3], v1.x, v1.y, o0.x, o0.y);
+      cmul(m[4], m[5], v0.x, v0.y, o1.x, o1.y);
+    }
+    vr[j] = o0.x;
+    vi[j] = o0.y;
+    vr[k] = o1.x;
+    vi[k] = o1.y;
+  }
+}
+
+// Controlled gates: pairs (j, j | T) with bit C of j set.  SWAP: pairs
+// (j, j ^ C ^ T) with (bit C, bit T) of j = (1, 0).
+template <int C, int T, int CLS>
+__device__ __forceinline__ void r_cu(double (&vr)[kGA], double (&vi)[kGA], const double *m) {
+#pragma unroll
+  for (int j = 0; j < kGA; ++j) {
+    if (!(j & (1 << C)) || (j & (1 << T))) continue;
+    if constexpr (CLS == kOpSwap) {
+      const int k = (j ^ (1 << C)) | (1 << T);
+      const double ar = vr[j], ai = vi[j];
+      vr[j] = vr[k];
+      vi[j] = vi[k];
+      vr[k] = ar;
+      vi[k] = ai;
+    } else {
+      const int k = j | (1 << T);
+      const double2 v0 = make_double2(vr[j], vi[j]), v1 = make_double2(vr[k], vi[k]);
+      if constexpr (CLS == kOpCuDense) {
+        double2 o0, o1;
+        row2(m[0], m[1], m[2], m[3], v0.x, v0.y, v1.x, v1.y, o0.x, o0.y);
+        row2(m[4], m[5], m[6], m[7], v0.x, v0.y, v1.x, v1.y, o1.x, o1.y);
+        vr[j] = o0.x;
+        vi[j] = o0.y;
+        vr[k] = o1.x;
+        vi[k] = o1.y;
+      } else if constexpr (CLS == kOpCnot) {
+        vr[j] = v1.x;
+        vi[j] = v1.y;
+        vr[k] = v0.x;
+        vi[k] = v0.y;
+      } else {  // kOpCPhase
+        double2 o1;
+        cmul(m[6], m[7], v1.x, v1.y, o1.x, o1.y);
+        vr[k] = o1.x;
+        vi[k] = o1.y;
+      }
+    }
+  }
+}
+
+template <int CLS>
+__device__ __forceinline__ void r_1q_at(double (&vr)[kGA], double (&vi)[kGA], int t, const double *m) {
+  switch (t) {
+    case 0: r_1q<0, CLS>(vr, vi, m); break;
+    case 1: r_1q<1, CLS>(vr, vi, m); break;
+    case 2: r_1q<2, CLS>(vr, vi, m); break;
+    default: r_1q<3, CLS>(vr, vi, m); break;
+  }
+}
+
+template <int CLS>
+__device__ __forceinline__ void r_cu_at(double (&vr)[kGA], double (&vi)[kGA], int c, int t,
+                                        const double *m) {
+  switch (c * kGR + t) {
+    case 1: r_cu<0, 1, CLS>(vr, vi, m); break;
+    case 2: r_cu<0, 2, CLS>(vr, vi, m); break;
+    case 3: r_cu<0, 3, CLS>(vr, vi, m); break;
+    case 4: r_cu<1, 0, CLS>(vr, vi, m); break;
+    case 6: r_cu<1, 2, CLS>(vr, vi, m); break;
+    case 7: r_cu<1, 3, CLS>(vr, vi, m); break;
+    case 8: r_cu<2, 0, CLS>(vr, vi, m); break;
+    case 9: r_cu<2, 1, CLS>(vr, vi, m); break;
+    case 11: r_cu<2, 3, CLS>(vr, vi, m); break;
+    case 12: r_cu<3, 0, CLS>(vr, vi, m); break;
+    case 13: r_cu<3, 1, CLS>(vr, vi, m); break;
+    default: r_cu<3, 2, CLS>(vr, vi, m); break;
+  }
+}
+
+__device__ __forceinline__ void r_gate(double (&vr)[kGA], double (&vi)[kGA], const BGate *__restrict__ g) {
+  double m[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) m[i] = g->m[i];
+  const int t = g->t, c = g->c;
+  switch (g->cls) {
+    case kOp1Dense: r_1q_at<kOp1Dense>(vr, vi, t, m); break;
+    case kOp1Phase: r_1q_at<kOp1Phase>(vr, vi, t, m); break;
+    case kOp1Diag: r_1q_at<kOp1Diag>(vr, vi, t, m); break;
+    case kOp1Real: r_1q_at<kOp1Real>(vr, vi, t, m); break;
+    case kOp1Anti: r_1q_at<kOp1Anti>(vr, vi, t, m); break;
+    case kOpCuDense: r_cu_at<kOpCuDense>(vr, vi, c, t, m); break;
+    case kOpCnot: r_cu_at<kOpCnot>(vr, vi, c, t, m); break;
+    case kOpCPhase: r_cu_at<kOpCPhase>(vr, vi, c, t, m); break;
+    default: r_cu_at<kOpSwap>(vr, vi, c, t, m); break;
+  }
+}
+
+// One group on a tile of 2^tb (>= kGR) amplitudes: thread `col` takes the
+// column of group-bit patterns at the other tile bits = col.
+template <int NTH>
+__device__ void tile_group(double2 *L, int tb, const BGroup *__restrict__ gp,
+                           const BGate *__restrict__ gates) {
+  const int ncols = 1 << (tb - kGR);
+  const int q0 = gp->qpos[0], q1 = gp->qpos[1], q2 = gp->qpos[2], q3 = gp->qpos[3];
+  const int ng = gp->ngates;
+  const BGate *g0 = gates + gp->first_gate;
+  int rs[kGA];
+#pragma unroll
+  for (int j = 0; j < kGA; ++j) rs[j] = gp->rowslot[j];
+  for (int col = threadIdx.x; col < ncols; col += NTH) {
+    const int base = dswz(bins0(bins0(bins0(bins0(col, q0), q1), q2), q3));
+    double vr[kGA], vi[kGA];
+#pragma unroll
+    for (int j = 0; j < kGA; ++j) {
+      const double2 x = L[base ^ rs[j]];
+      vr[j] = x.x;
+      vi[j] = x.y;
+    }
+    for (int g = 0; g < ng; ++g) r_gate(vr, vi, g0 + g);
+#pragma unroll
+    for (int j = 0; j < kGA; ++j) L[base ^ rs[j]] = make_double2(vr[j], vi[j]);
+  }
+}
+
+// complex128 tile <-> HBM: element e <-> G[base | deposit(e)].  Loads go
+// out kIoU per thread at a time (8 x 16 bytes in flight per lane), so one
+// work-group keeps 64 KiB of a tile in flight.
+constexpr int kIoU = 8;
+
 template <int NTH, bool LOAD>
 __device__ void tile_io_c128(double2 *L, double2 *G, uint64_t base,
                              const int *sb, int tb) {
-  constexpr int LT = __builtin_ctz(NTH), U = 4;
+  constexpr int LT = __builtin_ctz(NTH);
   const int tid = threadIdx.x, tile = 1 << tb;
   const uint64_t dt = base | deposit(uint32_t(tid), sb, 0, tb < LT ? tb : LT);
-  for (int e0 = 0; e0 < tile; e0 += U * NTH) {
-    double2 v[U];
-    uint64_t a[U];
+  int e0 = 0;
+  for (; e0 + kIoU * NTH <= tile; e0 += kIoU * NTH) {
+    uint64_t a[kIoU];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int e = e0 + u * NTH + tid;
-      a[u] = dt | deposit(uint32_t(e), sb, LT, tb);
-      if (LOAD && e < tile) v[u] = G[a[u]];
-    }
+    for (int u = 0; u < kIoU; ++u)
+      a[u] = dt | deposit(uint32_t(e0 + u * NTH + tid), sb, LT, tb);
+    if (LOAD) {
+      double2 v[kIoU];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int e = e0 + u * NTH + tid;
-      if (e >= tile) continue;
-      if (LOAD) L[e] = v[u];
-      else G[a[u]] = L[e];
+      for (int u = 0; u < kIoU; ++u) v[u] = G[a[u]];
+#pragma unroll
+      for (int u = 0; u < kIoU; ++u) L[dswz(e0 + u * NTH + tid)] = v[u];
+    } else {
+#pragma unroll
+      for (int u = 0; u < kIoU; ++u) G[a[u]] = L[dswz(e0 + u * NTH + tid)];
     }
+  }
+  for (int e = e0 + tid; e < tile; e += NTH) {
+    const uint64_t a = dt | deposit(uint32_t(e), sb, LT, tb);
+    if (LOAD) L[dswz(e)] = G[a];
+    else G[a] = L[dswz(e)];
   }
 }
 
@@ -236,6 +428,18 @@ __device__ void tile_block(char *Lb, int tb, const BBlock *__restrict__ bp,
   for (int q = 0; q < K; ++q) tpos[q] = bp->tpos[q];
 #pragma unroll
   for (int s = 0; s < R; ++s) rs[s] = bp->rowslot[s];
+  // slots of the output rows of this lane: accumulators 4q, 4q+1 hold row
+  // mt*16 + 4q + 2h, 4q+2, 4q+3 the next row (lane-dependent, so loaded per
+  // lane rather than picked from a private array)
+  int os0[MT][4], os1[MT][4];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int r0 = mt * 16 + 4 * q + 2 * h;
+      os0[mt][q] = r0 < R ? bp->rowslot[r0] : 0;
+      os1[mt][q] = r0 + 1 < R ? bp->rowslot[r0 + 1] : 0;
+    }
   const int ncols = 1 << (tb - K);
   for (int c0 = wave * 32; c0 < ncols; c0 += NW * 32) {
     const int cc = c0 + l32;
@@ -262,12 +466,14 @@ __device__ void tile_block(char *Lb, int tb, const BBlock *__restrict__ bp,
     for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int r0 = mt * 16 + 4 * q + 2 * h;
+        const int ra = mt * 16 + 4 * q;
+        if (ra >= R) continue;
+        const int r0 = ra + 2 * h;
         if (r0 < R)
-          *reinterpret_cast<float2 *>(Lb + (cs ^ rs[r0])) =
+          *reinterpret_cast<float2 *>(Lb + (cs ^ os0[mt][q])) =
               make_float2(acc[mt][4 * q], acc[mt][4 * q + 1]);
         if (r0 + 1 < R)
-          *reinterpret_cast<float2 *>(Lb + (cs ^ rs[r0 + 1])) =
+          *reinterpret_cast<float2 *>(Lb + (cs ^ os1[mt][q])) =
               make_float2(acc[mt][4 * q + 2], acc[mt][4 * q + 3]);
       }
   }
@@ -289,38 +495,58 @@ __device__ void tile_block_any(char *Lb, int tb, const BBlock *__restrict__ bp,
 // index bit 0, so a pair is one 16-byte access, and swizzled slots
 // bswz(e), bswz(e) ^ 1.
 template <int NTH, bool LOAD>
+__device__ __forceinline__ void pair_io_c64(char *Lb, float2 *G, uint64_t a,
+                                            int p) {
+  float2 *s0 = reinterpret_cast<float2 *>(Lb + 8 * bswz(2 * p));
+  float2 *s1 = reinterpret_cast<float2 *>(Lb + 8 * (bswz(2 * p) ^ 1));
+  if (LOAD) {
+    const bf4v v = *reinterpret_cast<const bf4v *>(G + a);
+    *s0 = make_float2(v.x, v.y);
+    *s1 = make_float2(v.z, v.w);
+  } else {
+    const float2 x = *s0, y = *s1;
+    const bf4v w = {x.x, x.y, y.x, y.y};
+    *reinterpret_cast<bf4v *>(G + a) = w;
+  }
+}
+
+template <int NTH, bool LOAD>
 __device__ void tile_io_c64(char *Lb, float2 *G, uint64_t base, const int *sb,
                             int tb) {
-  constexpr int LT = __builtin_ctz(NTH), U = 4;
+  constexpr int LT = __builtin_ctz(NTH);
   const int tid = threadIdx.x, npair = 1 << (tb - 1);
   const int jt = tb - 1 < LT ? tb - 1 : LT;  // tile bits 1..jt from tid
   const uint64_t dt = base | deposit(uint32_t(tid) << 1, sb, 1, 1 + jt);
-  for (int p0 = 0; p0 < npair; p0 += U * NTH) {
-    bf4v v[U];
-    uint64_t a[U];
+  int p0 = 0;
+  for (; p0 + 4 * NTH <= npair; p0 += 4 * NTH) {
+    const int pa = p0 + tid, pb = pa + NTH, pc = pb + NTH, pd = pc + NTH;
+    const uint64_t aa = dt | deposit(uint32_t(pa) << 1, sb, 1 + LT, tb);
+    const uint64_t ab = dt | deposit(uint32_t(pb) << 1, sb, 1 + LT, tb);
+    const uint64_t ac = dt | deposit(uint32_t(pc) << 1, sb, 1 + LT, tb);
+    const uint64_t ad = dt | deposit(uint32_t(pd) << 1, sb, 1 + LT, tb);
+    if (LOAD) {
+      const bf4v va = *reinterpret_cast<const bf4v *>(G + aa);
+      const bf4v vb = *reinterpret_cast<const bf4v *>(G + ab);
+      const bf4v vc = *reinterpret_cast<const bf4v *>(G + ac);
+      const bf4v vd = *reinterpret_cast<const bf4v *>(G + ad);
+      const bf4v vv[4] = {va, vb, vc, vd};
+      const int pp[4] = {pa, pb, pc, pd};
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int p = p0 + u * NTH + tid;
-      a[u] = dt | deposit(uint32_t(p) << 1, sb, 1 + LT, tb);
-      if (LOAD && p < npair)
-        v[u] = *reinterpret_cast<const bf4v *>(G + a[u]);
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int p = p0 + u * NTH + tid;
-      if (p >= npair) continue;
-      float2 *s0 = reinterpret_cast<float2 *>(Lb + 8 * bswz(2 * p));
-      float2 *s1 = reinterpret_cast<float2 *>(Lb + 8 * (bswz(2 * p) ^ 1));
-      if (LOAD) {
-        *s0 = make_float2(v[u].x, v[u].y);
-        *s1 = make_float2(v[u].z, v[u].w);
-      } else {
-        const float2 x = *s0, y = *s1;
-        const bf4v w = {x.x, x.y, y.x, y.y};
-        *reinterpret_cast<bf4v *>(G + a[u]) = w;
+      for (int u = 0; u < 4; ++u) {
+        float2 *s0 = reinterpret_cast<float2 *>(Lb + 8 * bswz(2 * pp[u]));
+        float2 *s1 = reinterpret_cast<float2 *>(Lb + 8 * (bswz(2 * pp[u]) ^ 1));
+        *s0 = make_float2(vv[u].x, vv[u].y);
+        *s1 = make_float2(vv[u].z, vv[u].w);
       }
+    } else {
+      pair_io_c64<NTH, false>(Lb, G, aa, pa);
+      pair_io_c64<NTH, false>(Lb, G, ab, pb);
+      pair_io_c64<NTH, false>(Lb, G, ac, pc);
+      pair_io_c64<NTH, false>(Lb, G, ad, pd);
     }
   }
+  for (int p = p0 + tid; p < npair; p += NTH)
+    pair_io_c64<NTH, LOAD>(Lb, G, dt | deposit(uint32_t(p) << 1, sb, 1 + LT, tb), p);
 }
 
 // ---- FP64 norm and sampling ------------------------------------------------------
@@ -406,8 +632,9 @@ __device__ void sample_state(const P &prob, int n, double *tot, double *inc,
 // ---- one circuit per work-group ---------------------------------------------------
 
 template <int NTH>
-__device__ void run_c128(const BCirc &c, const BPass *__restrict__ passes,
-                         const BGate *__restrict__ gates, char *states,
+__device__ void run_c128(const BCirc c, const BPass *__restrict__ passes,
+                         const BGate *__restrict__ gates,
+                         const BGroup *__restrict__ groups, char *states,
                          char *lds, double *tot, double *inc, double *red,
                          const double *__restrict__ u, long *__restrict__ out,
                          uint64_t shots, int keep) {
@@ -432,7 +659,10 @@ __device__ void run_c128(const BCirc &c, const BPass *__restrict__ passes,
       __syncthreads();
       if (p > 0 || t == 0) {
         for (int o = 0; o < ps.nops; ++o) {
-          tile_gate<NTH>(L, tb, &gates[ps.first_op + o]);
+          if (tb >= kGR)
+            tile_group<NTH>(L, tb, &groups[ps.first_op + o], gates);
+          else
+            tile_gate<NTH>(L, tb, &gates[ps.first_op + o]);
           __syncthreads();
         }
       }
@@ -443,13 +673,13 @@ __device__ void run_c128(const BCirc &c, const BPass *__restrict__ passes,
     }
   }
   if (resident && keep && G) {
-    for (int e = tid; e < tile; e += NTH) G[e] = L[e];
+    for (int e = tid; e < tile; e += NTH) G[e] = L[dswz(e)];
     __syncthreads();
   }
   (void)red;
   if (shots == 0) return;
   if (resident) {
-    sample_state<NTH>([&](uint64_t i) { return norm2(L[i]); }, n, tot, inc,
+    sample_state<NTH>([&](uint64_t i) { return norm2(L[dswz(int(i))]); }, n, tot, inc,
                       u + c.shot_off, out + c.shot_off, shots);
   } else {
     const double2 *Gc = G;
@@ -459,7 +689,7 @@ __device__ void run_c128(const BCirc &c, const BPass *__restrict__ passes,
 }
 
 template <int NTH>
-__device__ void run_c64(const BCirc &c, const BPass *__restrict__ passes,
+__device__ void run_c64(const BCirc c, const BPass *__restrict__ passes,
                         const BBlock *__restrict__ blocks,
                         const float *__restrict__ wbuf, char *states, char *lds,
                         double *tot, double *inc, double *red,
@@ -545,9 +775,10 @@ template <int NTH>
 __global__ __launch_bounds__(NTH) void k_batch(
     const BCirc *__restrict__ circs, const int *__restrict__ order, int nc,
     const BPass *__restrict__ passes, const BGate *__restrict__ gates,
-    const BBlock *__restrict__ blocks, const float *__restrict__ wbuf,
-    char *states, const double *__restrict__ u, long *__restrict__ out,
-    uint64_t shots, unsigned *queue, int tile_bytes, int nch_cap, int keep) {
+    const BGroup *__restrict__ groups, const BBlock *__restrict__ blocks,
+    const float *__restrict__ wbuf, char *states, const double *__restrict__ u,
+    long *__restrict__ out, uint64_t shots, unsigned *queue, int tile_bytes,
+    int nch_cap, int keep) {
   extern __shared__ double4 smem4[];
   char *lds = reinterpret_cast<char *>(smem4);
   double *tot = reinterpret_cast<double *>(lds + tile_bytes);
@@ -562,8 +793,8 @@ __global__ __launch_bounds__(NTH) void k_batch(
     if (i >= nc) break;
     const BCirc c = circs[order[i]];
     if (c.prec == QCS_HIP_C128)
-      run_c128<NTH>(c, passes, gates, states, lds, tot, inc, red, u, out,
-                    shots, keep);
+      run_c128<NTH>(c, passes, gates, groups, states, lds, tot, inc, red, u,
+                    out, shots, keep);
     else
       run_c64<NTH>(c, passes, blocks, wbuf, states, lds, tot, inc, red, u,
                    out, shots, keep);
@@ -615,7 +846,7 @@ struct qcs_hip_batch {
 namespace qcs {
 
 struct BatchLayout {
-  size_t circ, order, pass, gate, block, hgate, hblock, u, total;
+  size_t circ, order, pass, gate, group, block, hgate, hblock, u, total;
 };
 
 static size_t align_up(size_t x) { return (x + kBAlign - 1) / kBAlign * kBAlign; }
@@ -660,8 +891,7 @@ static int validate_batch(const qcs_hip_batch_circuit *cs, int nc,
                           const qcs_hip_pass *passes, int np,
                           const qcs_hip_block *blocks, int nb,
                           const qcs_hip_gate *gates, int ng, long shots,
-                          const double *u, long *out, size_t *n_bgate,
-                          size_t *n_bblock) {
+                          const double *u, long *out) {
   if (nc < 0 || np < 0 || nb < 0 || ng < 0 || shots < 0)
     return fail(QCS_HIP_EINVAL, "negative count");
   if ((nc > 0 && !cs) || (np > 0 && !passes) || (nb > 0 && !blocks) ||
@@ -671,7 +901,6 @@ static int validate_batch(const qcs_hip_batch_circuit *cs, int nc,
     return fail(QCS_HIP_EINVAL, "null uniforms or output");
   if (shots > (long(1) << 40) / (nc > 0 ? nc : 1))
     return fail(QCS_HIP_EINVAL, "too many shots");
-  *n_bgate = *n_bblock = 0;
   for (int i = 0; i < nc; ++i) {
     const qcs_hip_batch_circuit &c = cs[i];
     if (c.num_qubits < 1 || c.num_qubits > kBMaxN)
@@ -715,7 +944,6 @@ static int validate_batch(const qcs_hip_batch_circuit *cs, int nc,
                !((mask >> gt.control) & 1)))
             return fail(QCS_HIP_EINVAL, "pass %d: gate %d: bad control", p, g);
         }
-        *n_bgate += size_t(ps.num_blocks);
       } else {
         if (ps.first_block + ps.num_blocks > nb)
           return fail(QCS_HIP_EINVAL, "pass %d: block range outside the block array", p);
@@ -728,7 +956,6 @@ static int validate_batch(const qcs_hip_batch_circuit *cs, int nc,
             if (!((mask >> blocks[bi].targets[q]) & 1))
               return fail(QCS_HIP_EINVAL, "block %d: target outside the pass bits", bi);
         }
-        *n_bblock += size_t(ps.num_blocks);
       }
     }
   }
@@ -741,19 +968,76 @@ static int position_in(const qcs_hip_pass &ps, int q) {
   return -1;
 }
 
+// grouped: the gate runs in a register group, where the dense class splits
+// further (diagonal, real, anti-diagonal); elsewhere it keeps the classes of
+// the per-gate kernels
+static int gate_class(const qcs_hip_gate &g, bool grouped) {
+  const double *m = g.m;
+  if (g.kind == QCS_HIP_G_SWAP) return kOpSwap;
+  if (g.kind == QCS_HIP_G_CU)
+    return is_x(m) ? kOpCnot : (is_phase1(m) ? kOpCPhase : kOpCuDense);
+  if (is_phase1(m)) return kOp1Phase;
+  if (!grouped) return kOp1Dense;
+  if (m[2] == 0.0 && m[3] == 0.0 && m[4] == 0.0 && m[5] == 0.0) return kOp1Diag;
+  if (m[1] == 0.0 && m[3] == 0.0 && m[5] == 0.0 && m[7] == 0.0) return kOp1Real;
+  if (m[0] == 0.0 && m[1] == 0.0 && m[6] == 0.0 && m[7] == 0.0) return kOp1Anti;
+  return kOp1Dense;
+}
+
 static BGate make_bgate(const qcs_hip_pass &ps, const qcs_hip_gate &g) {
   BGate o;
   std::memset(&o, 0, sizeof(o));
   o.t = position_in(ps, g.target);
   o.c = g.kind == QCS_HIP_G_1Q ? 0 : position_in(ps, g.control);
   std::memcpy(o.m, g.m, sizeof(o.m));
-  if (g.kind == QCS_HIP_G_1Q)
-    o.cls = is_phase1(g.m) ? kOp1Phase : kOp1Dense;
-  else if (g.kind == QCS_HIP_G_SWAP)
-    o.cls = kOpSwap;
-  else
-    o.cls = is_x(g.m) ? kOpCnot : (is_phase1(g.m) ? kOpCPhase : kOpCuDense);
+  o.cls = gate_class(g, false);
   return o;
+}
+
+// Register groups of one complex128 pass (tile_bits >= kGR): maximal runs of
+// consecutive gates on at most kGR tile positions, padded with the lowest
+// unused positions.
+static void make_groups(const qcs_hip_pass &ps, const qcs_hip_gate *gates,
+                        std::vector<BGroup> *groups, std::vector<BGate> *bg) {
+  const int end = ps.first_block + ps.num_blocks;
+  int g = ps.first_block;
+  while (g < end) {
+    unsigned mask = 0;
+    const int start = g;
+    for (; g < end; ++g) {
+      unsigned gm = 1u << position_in(ps, gates[g].target);
+      if (gates[g].kind != QCS_HIP_G_1Q) gm |= 1u << position_in(ps, gates[g].control);
+      if (__builtin_popcount(mask | gm) > kGR) break;
+      mask |= gm;
+    }
+    for (int q = 0; __builtin_popcount(mask) < kGR; ++q) mask |= 1u << q;
+    BGroup grp;
+    std::memset(&grp, 0, sizeof(grp));
+    int local[32];
+    for (int q = 0, i = 0; q < 32; ++q)
+      if ((mask >> q) & 1u) {
+        local[q] = i;
+        grp.qpos[i++] = q;
+      }
+    for (int j = 0; j < kGA; ++j) {
+      int off = 0;
+      for (int i = 0; i < kGR; ++i)
+        if ((j >> i) & 1) off |= 1 << grp.qpos[i];
+      grp.rowslot[j] = dswz(off);
+    }
+    grp.first_gate = int(bg->size());
+    grp.ngates = g - start;
+    for (int x = start; x < g; ++x) {
+      BGate o;
+      std::memset(&o, 0, sizeof(o));
+      o.t = local[position_in(ps, gates[x].target)];
+      o.c = gates[x].kind == QCS_HIP_G_1Q ? 0 : local[position_in(ps, gates[x].control)];
+      std::memcpy(o.m, gates[x].m, sizeof(o.m));
+      o.cls = gate_class(gates[x], true);
+      bg->push_back(o);
+    }
+    groups->push_back(grp);
+  }
 }
 
 static BBlock make_bblock(const qcs_hip_pass &ps, const qcs_hip_block &bl,
@@ -782,6 +1066,7 @@ static int launch_batch(qcs_hip_batch *b, const BatchLayout &L, int nc,
                      reinterpret_cast<const int *>(d + L.order), nc,
                      reinterpret_cast<const BPass *>(d + L.pass),
                      reinterpret_cast<const BGate *>(d + L.gate),
+                     reinterpret_cast<const BGroup *>(d + L.group),
                      reinterpret_cast<const BBlock *>(d + L.block), b->d_w,
                      b->d_states, reinterpret_cast<const double *>(d + L.u),
                      b->d_out, shots, b->d_queue, tile_bytes, nch_cap,
@@ -846,28 +1131,67 @@ int batch_run(qcs_hip_batch *b, const qcs_hip_batch_circuit *cs, int nc,
               const qcs_hip_pass *passes, int np, const qcs_hip_block *blocks,
               int nb, const qcs_hip_gate *gates, int ng, long shots,
               const double *u, long *out) {
-  size_t n_bgate = 0, n_bblock = 0;
   int rc = validate_batch(cs, nc, passes, np, blocks, nb, gates, ng, shots, u,
-                          out, &n_bgate, &n_bblock);
+                          out);
   if (rc) return rc;
   if (nc == 0) return 0;
+  for (int i = 0; i < nc; ++i)
+    for (int p = cs[i].first_pass; p < cs[i].first_pass + cs[i].num_passes; ++p)
+      for (int j = 0; j < i; ++j)
+        if (p >= cs[j].first_pass && p < cs[j].first_pass + cs[j].num_passes &&
+            (cs[j].precision != cs[i].precision || cs[j].tile_bits != cs[i].tile_bits))
+          return fail(QCS_HIP_EINVAL, "pass %d shared by circuits of different kinds", p);
   QCS_HIP_TRY(hipSetDevice(b->device));
+
+  // device descriptors: passes, then their gates / register groups / blocks
+  std::vector<BPass> hp(static_cast<size_t>(np));
+  std::vector<BGate> hg;
+  std::vector<BGroup> hgrp;
+  std::vector<BBlock> hb;
+  std::vector<char> pass_done(size_t(np), 0);
+  for (int i = 0; i < nc; ++i) {
+    const qcs_hip_batch_circuit &c = cs[i];
+    for (int p = c.first_pass; p < c.first_pass + c.num_passes; ++p) {
+      if (pass_done[size_t(p)]) continue;
+      pass_done[size_t(p)] = 1;
+      const qcs_hip_pass &ps = passes[p];
+      BPass &dp = hp[size_t(p)];
+      std::memset(&dp, 0, sizeof(dp));
+      for (int j = 0; j < ps.tile_bits; ++j) dp.sbits[j] = ps.bits[j];
+      if (c.precision == QCS_HIP_C128 && c.tile_bits >= kGR) {
+        dp.first_op = int(hgrp.size());
+        make_groups(ps, gates, &hgrp, &hg);
+        dp.nops = int(hgrp.size()) - dp.first_op;
+      } else if (c.precision == QCS_HIP_C128) {
+        dp.first_op = int(hg.size());
+        for (int g = ps.first_block; g < ps.first_block + ps.num_blocks; ++g)
+          hg.push_back(make_bgate(ps, gates[g]));
+        dp.nops = ps.num_blocks;
+      } else {
+        dp.first_op = int(hb.size());
+        for (int bi = ps.first_block; bi < ps.first_block + ps.num_blocks; ++bi)
+          hb.push_back(make_bblock(ps, blocks[bi], bi, ps.tile_bits >= 5));
+        dp.nops = ps.num_blocks;
+      }
+    }
+  }
 
   // layout of the descriptor block (device and pinned staging alike)
   BatchLayout L;
   size_t off = 0;
   auto take = [&](size_t bytes) {
     const size_t at = off;
-    off = align_up(off + bytes);
+    off = align_up(off + (bytes ? bytes : 1));
     return at;
   };
   L.circ = take(sizeof(BCirc) * size_t(nc));
   L.order = take(sizeof(int) * size_t(nc));
   L.pass = take(sizeof(BPass) * size_t(np));
-  L.gate = take(sizeof(BGate) * (n_bgate ? n_bgate : 1));
-  L.block = take(sizeof(BBlock) * (n_bblock ? n_bblock : 1));
-  L.hgate = take(sizeof(qcs_hip_gate) * size_t(ng > 0 ? ng : 1));
-  L.hblock = take(sizeof(qcs_hip_block) * size_t(nb > 0 ? nb : 1));
+  L.gate = take(sizeof(BGate) * hg.size());
+  L.group = take(sizeof(BGroup) * hgrp.size());
+  L.block = take(sizeof(BBlock) * hb.size());
+  L.hgate = take(sizeof(qcs_hip_gate) * size_t(ng));
+  L.hblock = take(sizeof(qcs_hip_block) * size_t(nb));
   L.u = take(sizeof(double) * size_t(nc) * size_t(shots));
   L.total = off;
 
@@ -932,11 +1256,6 @@ int batch_run(qcs_hip_batch *b, const qcs_hip_batch_circuit *cs, int nc,
   char *h = b->h_stage;
   BCirc *hc = reinterpret_cast<BCirc *>(h + L.circ);
   int *ho = reinterpret_cast<int *>(h + L.order);
-  BPass *hp = reinterpret_cast<BPass *>(h + L.pass);
-  BGate *hg = reinterpret_cast<BGate *>(h + L.gate);
-  BBlock *hb = reinterpret_cast<BBlock *>(h + L.block);
-  size_t ig = 0, ib = 0;
-  std::vector<char> pass_done(size_t(np), 0);
   for (int i = 0; i < nc; ++i) {
     const qcs_hip_batch_circuit &c = cs[i];
     BCirc &d = hc[i];
@@ -948,35 +1267,12 @@ int batch_run(qcs_hip_batch *b, const qcs_hip_batch_circuit *cs, int nc,
     d.first_pass = c.first_pass;
     d.state_off = slot[i];
     d.shot_off = uint64_t(i) * uint64_t(shots);
-    for (int p = c.first_pass; p < c.first_pass + c.num_passes; ++p) {
-      // a pass shared by several circuits is written once; its ops then
-      // follow the first circuit's precision, which validation has checked
-      // against each of them
-      if (pass_done[size_t(p)]) continue;
-      pass_done[size_t(p)] = 1;
-      const qcs_hip_pass &ps = passes[p];
-      BPass &dp = hp[p];
-      std::memset(&dp, 0, sizeof(dp));
-      for (int j = 0; j < ps.tile_bits; ++j) dp.sbits[j] = ps.bits[j];
-      dp.nops = ps.num_blocks;
-      if (c.precision == QCS_HIP_C128) {
-        dp.first_op = int(ig);
-        for (int g = ps.first_block; g < ps.first_block + ps.num_blocks; ++g)
-          hg[ig++] = make_bgate(ps, gates[g]);
-      } else {
-        dp.first_op = int(ib);
-        for (int bi = ps.first_block; bi < ps.first_block + ps.num_blocks; ++bi)
-          hb[ib++] = make_bblock(ps, blocks[bi], bi, ps.tile_bits >= 5);
-      }
-    }
   }
-  // a pass used by circuits of both precisions cannot be encoded twice
-  for (int i = 0; i < nc; ++i)
-    for (int p = cs[i].first_pass; p < cs[i].first_pass + cs[i].num_passes; ++p)
-      for (int j = 0; j < i; ++j)
-        if (p >= cs[j].first_pass && p < cs[j].first_pass + cs[j].num_passes &&
-            cs[j].precision != cs[i].precision)
-          return fail(QCS_HIP_EINVAL, "pass %d shared by circuits of both precisions", p);
+  if (np > 0) std::memcpy(h + L.pass, hp.data(), sizeof(BPass) * hp.size());
+  if (!hg.empty()) std::memcpy(h + L.gate, hg.data(), sizeof(BGate) * hg.size());
+  if (!hgrp.empty())
+    std::memcpy(h + L.group, hgrp.data(), sizeof(BGroup) * hgrp.size());
+  if (!hb.empty()) std::memcpy(h + L.block, hb.data(), sizeof(BBlock) * hb.size());
   if (ng > 0) std::memcpy(h + L.hgate, gates, sizeof(qcs_hip_gate) * size_t(ng));
   if (nb > 0) std::memcpy(h + L.hblock, blocks, sizeof(qcs_hip_block) * size_t(nb));
   if (shots > 0)
@@ -996,9 +1292,9 @@ int batch_run(qcs_hip_batch *b, const qcs_hip_batch_circuit *cs, int nc,
   const int grid = std::min(nc, per_cu * b->num_cus);
 
   const bool use_graph = (b->flags & QCS_HIP_BATCH_GRAPH) != 0;
-  std::vector<long> key = {b->generation, nc, np, nb, ng, long(n_bgate),
-                           long(n_bblock), shots, b->nth, lds_bytes,
-                           grid, long(L.total), long(states_bytes)};
+  std::vector<long> key = {b->generation, nc, np, nb, ng, long(hg.size()),
+                           long(hgrp.size()), long(hb.size()), shots, b->nth,
+                           lds_bytes, grid, long(L.total), long(states_bytes)};
   QCS_HIP_TRY(hipEventRecord(b->ev[0], b->stream));
   b->replayed = 0;
   if (use_graph) {

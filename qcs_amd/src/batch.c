@@ -4,9 +4,10 @@
  * A QCS_DEFERRED circuit records its gates and owns no device state.  A
  * batch run plans every circuit on the host and hands the plans to the
  * backend's persistent kernel in one call (qcs_hip_batch_run):
- *   - complex128 circuits: the gates in order, cut into passes of
- *     consecutive gates whose qubits fit one LDS tile (qcs_gate_pass_plan);
- *     a circuit of at most tile_bits qubits is one pass and stays on chip;
+ *   - complex128 circuits: the gates in order (exact gates such as CNOT
+ *     moved up, qcs_gate_hoist_exact), cut into passes of consecutive gates
+ *     whose qubits fit one LDS tile (qcs_gate_pass_plan); a circuit of at
+ *     most tile_bits qubits is one pass and stays on chip;
  *   - QCS_FUSED_FP32 circuits: fused blocks of <= k qubits (qcs_fusion_plan,
  *     or qcs_pass_plan when the state is larger than a tile).
  * Uniforms come from rand() circuit after circuit, shot after shot, which is
@@ -19,7 +20,10 @@
 
 #include "qcs_internal.h"
 
-#define QCS_BATCH_LOW_BITS 6
+/* lowest qubits every streamed tile holds: 2^4 complex128 or 2^5 complex64
+ * amplitudes = 256-byte runs in HBM */
+#define QCS_BATCH_LOW_BITS_C128 4
+#define QCS_BATCH_LOW_BITS_C64 5
 
 struct s_q_batch {
   qcs_hip_batch *dev;
@@ -143,14 +147,22 @@ static int append_circuit(batch_plan *P, const t_q_batch *b, t_q_circuit *c,
       }
     } else {
       /* room for a k-qubit block beside the low qubits */
-      int low = QCS_BATCH_LOW_BITS;
+      int low = QCS_BATCH_LOW_BITS_C64;
       if (low > tb - 2) low = tb - 2;
       rc = qcs_pass_plan(gates, ng, n, tb, low, k, INT_MAX, &passes, &np,
                          &blocks, &nb, &ordered);
     }
     gates = ordered;
   } else {
-    rc = qcs_gate_pass_plan(gates, ng, n, tb, QCS_BATCH_LOW_BITS, &passes, &np);
+    /* exact gates (CNOT, X, Z, SWAP, ...) move up to their qubits' previous
+       gate, which gives the same amplitudes and far fewer passes (the
+       layered circuit at 20 qubits: 2.05 instead of 5.5 per layer) */
+    rc = qcs_gate_hoist_exact(gates, ng, n, &ordered);
+    if (rc == 0) {
+      gates = ordered;
+      rc = qcs_gate_pass_plan(gates, ng, n, tb, QCS_BATCH_LOW_BITS_C128,
+                              &passes, &np);
+    }
   }
   if (rc != 0) goto out;
   rc = -1;

@@ -448,3 +448,69 @@ int qcs_gate_pass_plan(const qcs_hip_gate *gates, int ng, int n, int tile_bits,
   *npasses_out = np;
   return 0;
 }
+
+/* a gate whose matrix is a permutation with entries in {+-1, +-i}: applying
+ * it only moves amplitudes and flips signs / swaps re and im, with no
+ * rounding, so it commutes bit-exactly with any gate on other qubits */
+static int is_unit(double re, double im) {
+  return (im == 0.0 && (re == 1.0 || re == -1.0)) ||
+         (re == 0.0 && (im == 1.0 || im == -1.0));
+}
+
+static int is_zero(double re, double im) { return re == 0.0 && im == 0.0; }
+
+int qcs_gate_is_exact(const qcs_hip_gate *g) {
+  const double *m = g->m;
+  if (g->kind == QCS_HIP_G_SWAP) return 1;
+  if (is_zero(m[2], m[3]) && is_zero(m[4], m[5]))
+    return is_unit(m[0], m[1]) && is_unit(m[6], m[7]);
+  if (is_zero(m[0], m[1]) && is_zero(m[6], m[7]))
+    return is_unit(m[2], m[3]) && is_unit(m[4], m[5]);
+  return 0;
+}
+
+int qcs_gate_hoist_exact(const qcs_hip_gate *gates, int ng, int n,
+                         qcs_hip_gate **ordered_out) {
+  int *bucket = NULL, *last = NULL, *start = NULL, *fill = NULL;
+  qcs_hip_gate *out = NULL;
+  int nbk = 1, i, q, rc = -1;
+  *ordered_out = NULL;
+  if (ng < 0 || n < 1) return -1;
+  bucket = (int *)malloc(((size_t)ng + 1) * sizeof(int));
+  last = (int *)calloc((size_t)n, sizeof(int));
+  out = (qcs_hip_gate *)malloc(((size_t)ng + 1) * sizeof(qcs_hip_gate));
+  if (!bucket || !last || !out) goto done;
+  /* bucket 0 holds exact gates before the first other gate; every other gate
+   * opens a bucket; an exact gate joins the latest bucket that touches one
+   * of its qubits */
+  for (i = 0; i < ng; i++) {
+    int qs[2], nq = gate_qubits(&gates[i], qs), j, b = 0;
+    for (j = 0; j < nq; j++) {
+      if (qs[j] < 0 || qs[j] >= n) goto done;
+      if (last[qs[j]] > b) b = last[qs[j]];
+    }
+    if (!qcs_gate_is_exact(&gates[i])) b = nbk++;
+    bucket[i] = b;
+    for (j = 0; j < nq; j++) last[qs[j]] = b;
+  }
+  start = (int *)calloc((size_t)nbk + 1, sizeof(int));
+  fill = (int *)calloc((size_t)nbk, sizeof(int));
+  if (!start || !fill) goto done;
+  for (i = 0; i < ng; i++) start[bucket[i] + 1]++;
+  for (q = 0; q < nbk; q++) start[q + 1] += start[q];
+  /* stable within a bucket: the opening gate first, then exact gates in
+   * circuit order */
+  for (i = 0; i < ng; i++) out[start[bucket[i]] + fill[bucket[i]]++] = gates[i];
+  rc = 0;
+done:
+  free(bucket);
+  free(last);
+  free(start);
+  free(fill);
+  if (rc != 0) {
+    free(out);
+    return rc;
+  }
+  *ordered_out = out;
+  return 0;
+}

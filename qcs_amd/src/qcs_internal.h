@@ -56,4 +56,15 @@ int qcs_pass_plan(const qcs_hip_gate *gates, int ng, int n, int tile_bits,
 int qcs_gate_pass_plan(const qcs_hip_gate *gates, int ng, int n, int tile_bits,
                        int low_bits, qcs_hip_pass **passes, int *npasses);
 
+/* 1 if the gate only permutes amplitudes and multiplies them by +-1 or
+ * +-i (X, Y, Z, CNOT, CZ, SWAP, ...): it then commutes bit-exactly with
+ * any gate on other qubits. */
+int qcs_gate_is_exact(const qcs_hip_gate *g);
+/* Reorder a gate list, moving every exact gate (qcs_gate_is_exact) to just
+ * after the latest earlier gate sharing a qubit with it; the other gates
+ * keep their order.  The result gives the same amplitudes as the original
+ * order (equal values).  *ordered is malloc'ed.  Returns 0 or -1. */
+int qcs_gate_hoist_exact(const qcs_hip_gate *gates, int ng, int n,
+                         qcs_hip_gate **ordered);
+
 #endif

@@ -4,6 +4,7 @@ whose qubits fit one tile; deferred circuits record without a device; a
 batch needs a device and says so."""
 import ctypes
 
+import numpy as np
 import pytest
 
 import qcs_amd
@@ -95,3 +96,82 @@ def test_batch_needs_device():
     h = ctypes.c_void_p()
     assert hip.qcs_hip_batch_create(ctypes.byref(h), 0) == -2
     assert not h.value
+
+
+def hoist(gates, ng, n):
+    _, qcs = qcs_amd.load()
+    po = ctypes.POINTER(HipGate)()
+    assert qcs.qcs_gate_hoist_exact(gates, ng, n, ctypes.byref(po)) == 0
+    return [po[i] for i in range(ng)]
+
+
+def apply_np(state, g, n):
+    """One gate with per-gate rounding: out0 = m00*v0 + m01*v1 (numpy)."""
+    m = np.array([[g.m[0] + 1j * g.m[1], g.m[2] + 1j * g.m[3]],
+                  [g.m[4] + 1j * g.m[5], g.m[6] + 1j * g.m[7]]])
+    idx = np.arange(1 << n)
+    tb = 1 << g.target
+    if g.kind == 2:
+        cb = 1 << g.control
+        sel = idx[((idx & tb) != 0) & ((idx & cb) == 0)]
+        out = state.copy()
+        out[sel], out[sel ^ tb ^ cb] = state[sel ^ tb ^ cb], state[sel]
+        return out
+    sel = idx[(idx & tb) == 0]
+    if g.kind == 1:
+        sel = sel[(sel & (1 << g.control)) != 0]
+    v0, v1 = state[sel], state[sel | tb]
+    out = state.copy()
+    out[sel] = m[0, 0] * v0 + m[0, 1] * v1
+    out[sel | tb] = m[1, 0] * v0 + m[1, 1] * v1
+    return out
+
+
+@pytest.mark.parametrize("n", [3, 6, 9])
+def test_hoisting_exact_gates_keeps_amplitudes(n):
+    _, qcs = qcs_amd.load()
+    kinds = [C.H, C.X, C.Y, C.Z, C.RY, C.RZ, C.CNOT, C.CPHASE, C.SWAP]
+    for seed in range(4):
+        ops = C.random_circuit(n, 80, 31 * n + seed, kinds=kinds)
+        gates = to_gates(ops)
+        out = hoist(gates, len(ops), n)
+        key = lambda g: (g.kind, g.target, g.control, tuple(g.m))  # noqa: E731
+        assert sorted(map(key, out)) == sorted(key(gates[i]) for i in range(len(ops)))
+        nonexact = [key(gates[i]) for i in range(len(ops))
+                    if not qcs.qcs_gate_is_exact(ctypes.byref(gates[i]))]
+        assert [key(g) for g in out if not qcs.qcs_gate_is_exact(ctypes.byref(g))] == nonexact
+        rng = np.random.default_rng(seed)
+        s0 = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+        a, b = s0.copy(), s0.copy()
+        for i in range(len(ops)):
+            a = apply_np(a, gates[i], n)
+        for g in out:
+            b = apply_np(b, g, n)
+        assert np.array_equal(a, b)
+
+
+def test_hoisting_cuts_layered_passes():
+    n, d = 20, 10
+    ops = C.layered(n, d, 1)
+    gates = to_gates(ops)
+    out = hoist(gates, len(ops), n)
+    arr = (HipGate * len(out))(*out)
+    _, qcs = qcs_amd.load()
+    pp, np_ = ctypes.POINTER(HipPass)(), ctypes.c_int()
+    assert qcs.qcs_gate_pass_plan(arr, len(out), n, 13, 4, ctypes.byref(pp),
+                                  ctypes.byref(np_)) == 0
+    hoisted = np_.value
+    passes, _ = gate_plan(ops, n, 13, 4)
+    assert hoisted * 2 < len(passes)
+    assert hoisted <= 2.2 * d
+
+
+def test_exact_gate_classes():
+    _, qcs = qcs_amd.load()
+    ops = C.Ops()
+    for k in (C.X, C.Y, C.Z, C.H, C.RZ):
+        ops.add(k, 0, -1, 0.3)
+    ops.add(C.CNOT, 1, 0).add(C.SWAP, 1, 0).add(C.CPHASE, 1, 0, 0.5)
+    g = to_gates(ops)
+    assert [qcs.qcs_gate_is_exact(ctypes.byref(g[i])) for i in range(len(ops))] == \
+        [1, 1, 1, 0, 0, 1, 1, 0]
