@@ -33,8 +33,12 @@
 // operations in the same order as reduce.hip, so the same state gives the
 // same indices.
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
+#include <thread>
 #include <vector>
 
 #include "device.h"
@@ -999,14 +1003,16 @@ static BGate make_bgate(const qcs_hip_pass &ps, const qcs_hip_gate &g) {
 // unused positions.
 static void make_groups(const qcs_hip_pass &ps, const qcs_hip_gate *gates,
                         std::vector<BGroup> *groups, std::vector<BGate> *bg) {
+  int pos[64];
+  for (int j = 0; j < ps.tile_bits; ++j) pos[ps.bits[j]] = j;
   const int end = ps.first_block + ps.num_blocks;
   int g = ps.first_block;
   while (g < end) {
     unsigned mask = 0;
     const int start = g;
     for (; g < end; ++g) {
-      unsigned gm = 1u << position_in(ps, gates[g].target);
-      if (gates[g].kind != QCS_HIP_G_1Q) gm |= 1u << position_in(ps, gates[g].control);
+      unsigned gm = 1u << pos[gates[g].target];
+      if (gates[g].kind != QCS_HIP_G_1Q) gm |= 1u << pos[gates[g].control];
       if (__builtin_popcount(mask | gm) > kGR) break;
       mask |= gm;
     }
@@ -1030,8 +1036,8 @@ static void make_groups(const qcs_hip_pass &ps, const qcs_hip_gate *gates,
     for (int x = start; x < g; ++x) {
       BGate o;
       std::memset(&o, 0, sizeof(o));
-      o.t = local[position_in(ps, gates[x].target)];
-      o.c = gates[x].kind == QCS_HIP_G_1Q ? 0 : local[position_in(ps, gates[x].control)];
+      o.t = local[pos[gates[x].target]];
+      o.c = gates[x].kind == QCS_HIP_G_1Q ? 0 : local[pos[gates[x].control]];
       std::memcpy(o.m, gates[x].m, sizeof(o.m));
       o.cls = gate_class(gates[x], true);
       bg->push_back(o);
@@ -1127,53 +1133,130 @@ static int enqueue_run(qcs_hip_batch *b, const BatchLayout &L, int nc, int nb,
   return 0;
 }
 
+// host threads for building descriptors: QCS_HOST_THREADS, else the
+// hardware's, at most 64
+static int host_threads() {
+  int t = int(std::thread::hardware_concurrency());
+  const char *env = std::getenv("QCS_HOST_THREADS");
+  if (env && std::atoi(env) > 0) t = std::atoi(env);
+  return std::max(1, std::min(t, 64));
+}
+
+static double host_ms() {
+  return std::chrono::duration<double, std::milli>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
 int batch_run(qcs_hip_batch *b, const qcs_hip_batch_circuit *cs, int nc,
               const qcs_hip_pass *passes, int np, const qcs_hip_block *blocks,
               int nb, const qcs_hip_gate *gates, int ng, long shots,
               const double *u, long *out) {
+  const double t0 = host_ms();
   int rc = validate_batch(cs, nc, passes, np, blocks, nb, gates, ng, shots, u,
                           out);
   if (rc) return rc;
   if (nc == 0) return 0;
+  // a pass may serve several circuits, all of one precision and tile size;
+  // the first circuit that names it owns it
+  std::vector<int> owner(static_cast<size_t>(np), -1);
   for (int i = 0; i < nc; ++i)
-    for (int p = cs[i].first_pass; p < cs[i].first_pass + cs[i].num_passes; ++p)
-      for (int j = 0; j < i; ++j)
-        if (p >= cs[j].first_pass && p < cs[j].first_pass + cs[j].num_passes &&
-            (cs[j].precision != cs[i].precision || cs[j].tile_bits != cs[i].tile_bits))
-          return fail(QCS_HIP_EINVAL, "pass %d shared by circuits of different kinds", p);
+    for (int p = cs[i].first_pass; p < cs[i].first_pass + cs[i].num_passes; ++p) {
+      const int j = owner[size_t(p)];
+      if (j < 0) owner[size_t(p)] = i;
+      else if (cs[j].precision != cs[i].precision || cs[j].tile_bits != cs[i].tile_bits)
+        return fail(QCS_HIP_EINVAL, "pass %d shared by circuits of different kinds", p);
+    }
   QCS_HIP_TRY(hipSetDevice(b->device));
 
-  // device descriptors: passes, then their gates / register groups / blocks
+  const double t1 = host_ms();
+  // device descriptors: passes, then their gates / register groups / blocks,
+  // built for contiguous ranges of circuits on host threads and joined
   std::vector<BPass> hp(static_cast<size_t>(np));
+  struct Part {
+    std::vector<BGate> g;
+    std::vector<BGroup> grp;
+    std::vector<BBlock> blk;
+    std::vector<int> own;  // passes built by this part
+  };
+  const int nparts = std::max(1, std::min(nc / 64 + 1, host_threads()));
+  std::vector<Part> parts(static_cast<size_t>(nparts));
+  auto build_part = [&](int t) {
+    Part &pt = parts[size_t(t)];
+    const int i0 = int(int64_t(nc) * t / nparts), i1 = int(int64_t(nc) * (t + 1) / nparts);
+    for (int i = i0; i < i1; ++i) {
+      const qcs_hip_batch_circuit &c = cs[i];
+      for (int p = c.first_pass; p < c.first_pass + c.num_passes; ++p) {
+        if (owner[size_t(p)] != i) continue;
+        const qcs_hip_pass &ps = passes[p];
+        BPass &dp = hp[size_t(p)];
+        std::memset(&dp, 0, sizeof(dp));
+        for (int j = 0; j < ps.tile_bits; ++j) dp.sbits[j] = ps.bits[j];
+        if (c.precision == QCS_HIP_C128 && c.tile_bits >= kGR) {
+          dp.first_op = int(pt.grp.size());
+          make_groups(ps, gates, &pt.grp, &pt.g);
+          dp.nops = int(pt.grp.size()) - dp.first_op;
+        } else if (c.precision == QCS_HIP_C128) {
+          dp.first_op = int(pt.g.size());
+          for (int g = ps.first_block; g < ps.first_block + ps.num_blocks; ++g)
+            pt.g.push_back(make_bgate(ps, gates[g]));
+          dp.nops = ps.num_blocks;
+        } else {
+          dp.first_op = int(pt.blk.size());
+          for (int bi = ps.first_block; bi < ps.first_block + ps.num_blocks; ++bi)
+            pt.blk.push_back(make_bblock(ps, blocks[bi], bi, ps.tile_bits >= 5));
+          dp.nops = ps.num_blocks;
+        }
+        pt.own.push_back(p);
+      }
+    }
+  };
+  {
+    std::vector<std::thread> th;
+    for (int t = 1; t < nparts; ++t) th.emplace_back(build_part, t);
+    build_part(0);
+    for (auto &x : th) x.join();
+  }
   std::vector<BGate> hg;
   std::vector<BGroup> hgrp;
   std::vector<BBlock> hb;
-  std::vector<char> pass_done(size_t(np), 0);
-  for (int i = 0; i < nc; ++i) {
-    const qcs_hip_batch_circuit &c = cs[i];
-    for (int p = c.first_pass; p < c.first_pass + c.num_passes; ++p) {
-      if (pass_done[size_t(p)]) continue;
-      pass_done[size_t(p)] = 1;
-      const qcs_hip_pass &ps = passes[p];
-      BPass &dp = hp[size_t(p)];
-      std::memset(&dp, 0, sizeof(dp));
-      for (int j = 0; j < ps.tile_bits; ++j) dp.sbits[j] = ps.bits[j];
-      if (c.precision == QCS_HIP_C128 && c.tile_bits >= kGR) {
-        dp.first_op = int(hgrp.size());
-        make_groups(ps, gates, &hgrp, &hg);
-        dp.nops = int(hgrp.size()) - dp.first_op;
-      } else if (c.precision == QCS_HIP_C128) {
-        dp.first_op = int(hg.size());
-        for (int g = ps.first_block; g < ps.first_block + ps.num_blocks; ++g)
-          hg.push_back(make_bgate(ps, gates[g]));
-        dp.nops = ps.num_blocks;
-      } else {
-        dp.first_op = int(hb.size());
-        for (int bi = ps.first_block; bi < ps.first_block + ps.num_blocks; ++bi)
-          hb.push_back(make_bblock(ps, blocks[bi], bi, ps.tile_bits >= 5));
-        dp.nops = ps.num_blocks;
-      }
+  {
+    size_t ng_all = 0, ngrp_all = 0, nb_all = 0;
+    for (const Part &pt : parts) {
+      ng_all += pt.g.size();
+      ngrp_all += pt.grp.size();
+      nb_all += pt.blk.size();
     }
+    hg.reserve(ng_all);
+    hgrp.reserve(ngrp_all);
+    hb.reserve(nb_all);
+  }
+  for (Part &pt : parts) {
+    const int og = int(hg.size()), ogrp = int(hgrp.size()), ob = int(hb.size());
+    for (int p : pt.own) {
+      const int i = owner[size_t(p)];
+      if (cs[i].precision == QCS_HIP_C128 && cs[i].tile_bits >= kGR)
+        hp[size_t(p)].first_op += ogrp;
+      else if (cs[i].precision == QCS_HIP_C128)
+        hp[size_t(p)].first_op += og;
+      else
+        hp[size_t(p)].first_op += ob;
+    }
+    for (BGroup &grp : pt.grp) grp.first_gate += og;
+    hg.insert(hg.end(), pt.g.begin(), pt.g.end());
+    hgrp.insert(hgrp.end(), pt.grp.begin(), pt.grp.end());
+    hb.insert(hb.end(), pt.blk.begin(), pt.blk.end());
+    Part().g.swap(pt.g);
+  }
+
+  // composition inputs: the blocks and only the gates they use (complex128
+  // circuits never reach the composer)
+  std::vector<qcs_hip_block> cblk(blocks, blocks + nb);
+  std::vector<qcs_hip_gate> cgate;
+  for (int bi = 0; bi < nb; ++bi) {
+    cblk[size_t(bi)].first_gate = int(cgate.size());
+    cgate.insert(cgate.end(), gates + blocks[bi].first_gate,
+                 gates + blocks[bi].first_gate + blocks[bi].num_gates);
   }
 
   // layout of the descriptor block (device and pinned staging alike)
@@ -1190,7 +1273,7 @@ int batch_run(qcs_hip_batch *b, const qcs_hip_batch_circuit *cs, int nc,
   L.gate = take(sizeof(BGate) * hg.size());
   L.group = take(sizeof(BGroup) * hgrp.size());
   L.block = take(sizeof(BBlock) * hb.size());
-  L.hgate = take(sizeof(qcs_hip_gate) * size_t(ng));
+  L.hgate = take(sizeof(qcs_hip_gate) * cgate.size());
   L.hblock = take(sizeof(qcs_hip_block) * size_t(nb));
   L.u = take(sizeof(double) * size_t(nc) * size_t(shots));
   L.total = off;
@@ -1273,8 +1356,9 @@ int batch_run(qcs_hip_batch *b, const qcs_hip_batch_circuit *cs, int nc,
   if (!hgrp.empty())
     std::memcpy(h + L.group, hgrp.data(), sizeof(BGroup) * hgrp.size());
   if (!hb.empty()) std::memcpy(h + L.block, hb.data(), sizeof(BBlock) * hb.size());
-  if (ng > 0) std::memcpy(h + L.hgate, gates, sizeof(qcs_hip_gate) * size_t(ng));
-  if (nb > 0) std::memcpy(h + L.hblock, blocks, sizeof(qcs_hip_block) * size_t(nb));
+  if (!cgate.empty())
+    std::memcpy(h + L.hgate, cgate.data(), sizeof(qcs_hip_gate) * cgate.size());
+  if (nb > 0) std::memcpy(h + L.hblock, cblk.data(), sizeof(qcs_hip_block) * size_t(nb));
   if (shots > 0)
     std::memcpy(h + L.u, u, sizeof(double) * size_t(nc) * size_t(shots));
   {
@@ -1285,6 +1369,7 @@ int batch_run(qcs_hip_batch *b, const qcs_hip_batch_circuit *cs, int nc,
     std::memcpy(ho, idx.data(), sizeof(int) * size_t(nc));
   }
 
+  const double t2 = host_ms();
   // persistent grid: every work-group resident at once
   int per_cu = 0;
   if ((rc = batch_occupancy(b->nth, lds_bytes, &per_cu))) return rc;
@@ -1292,7 +1377,7 @@ int batch_run(qcs_hip_batch *b, const qcs_hip_batch_circuit *cs, int nc,
   const int grid = std::min(nc, per_cu * b->num_cus);
 
   const bool use_graph = (b->flags & QCS_HIP_BATCH_GRAPH) != 0;
-  std::vector<long> key = {b->generation, nc, np, nb, ng, long(hg.size()),
+  std::vector<long> key = {b->generation, nc, np, nb, long(cgate.size()), long(hg.size()),
                            long(hgrp.size()), long(hb.size()), shots, b->nth,
                            lds_bytes, grid, long(L.total), long(states_bytes)};
   QCS_HIP_TRY(hipEventRecord(b->ev[0], b->stream));
@@ -1332,8 +1417,15 @@ int batch_run(qcs_hip_batch *b, const qcs_hip_batch_circuit *cs, int nc,
     QCS_HIP_TRY(hipEventElapsedTime(&ms, b->ev[1], b->ev[2]));
     b->kernel_ms = ms;
   }
+  const double t3 = host_ms();
   if (shots > 0)
     std::memcpy(out, b->h_out, sizeof(long) * size_t(nc) * size_t(shots));
+  const char *prof = std::getenv("QCS_BATCH_PROFILE");
+  if (prof && prof[0] && prof[0] != '0')
+    std::fprintf(stderr,
+                 "qcs_hip_batch_run: validate %.1f ms, descriptors %.1f ms, "
+                 "device %.1f ms (kernel %.1f), wall to sync %.1f ms\n",
+                 t1 - t0, t2 - t1, b->total_ms, b->kernel_ms, t3 - t2);
   b->last_n.assign(size_t(nc), 0);
   b->last_prec.assign(size_t(nc), 0);
   b->last_off = slot;

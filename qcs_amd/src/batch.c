@@ -13,10 +13,14 @@
  * Uniforms come from rand() circuit after circuit, shot after shot, which is
  * the stream qc_sample on each circuit in turn would draw (src/qcs.c:596).
  */
+#define _POSIX_C_SOURCE 200112L
 #include <limits.h>
+#include <time.h>
+#include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <unistd.h>
 
 #include "qcs_internal.h"
 
@@ -94,124 +98,149 @@ static int plan_fuse_k(const t_q_batch *b, const t_q_circuit *c, int tb) {
   return k;
 }
 
-/* growable arrays of the concatenated plans */
+/* the plan of one circuit, built independently of the others */
 typedef struct {
-  qcs_hip_batch_circuit *circ;
-  qcs_hip_pass *pass;
-  qcs_hip_block *block;
-  qcs_hip_gate *gate;
-  int np, nb, ng, cap_p, cap_b, cap_g;
-} batch_plan;
+  qcs_hip_pass *passes;
+  qcs_hip_block *blocks;
+  qcs_hip_gate *ordered; /* NULL: the circuit's recorded gates as they are */
+  int np, nb, tb, rc;
+} circuit_plan;
 
-static int reserve(void **p, int *cap, int need, size_t elem) {
-  void *q;
-  int c;
-  if (need <= *cap) return 0;
-  c = *cap ? *cap : 64;
-  while (c < need) {
-    if (c > INT_MAX / 2) return -1;
-    c *= 2;
-  }
-  q = realloc(*p, (size_t)c * elem);
-  if (!q) return -1;
-  *p = q;
-  *cap = c;
-  return 0;
-}
-
-static int append_circuit(batch_plan *P, const t_q_batch *b, t_q_circuit *c,
-                          int i) {
+static void plan_circuit(const t_q_batch *b, const t_q_circuit *c,
+                         circuit_plan *cp) {
   const int n = c->num_qubits, fused = (c->options & QCS_FUSED_FP32) != 0;
   const int tb = plan_tile_bits(b, fused, n);
-  qcs_hip_pass *passes = NULL;
-  qcs_hip_block *blocks = NULL;
-  qcs_hip_gate *ordered = NULL;
-  int np = 0, nb = 0, rc = 0, j, q;
   const qcs_hip_gate *gates = c->pending;
-  int ng = c->pending_size;
-
+  const int ng = c->pending_size;
+  int rc, q;
+  memset(cp, 0, sizeof(*cp));
+  cp->tb = tb;
   if (fused) {
     const int k = plan_fuse_k(b, c, tb);
     if (n <= tb) {
-      rc = qcs_fusion_plan(gates, ng, n, k, &blocks, &nb, &ordered);
+      rc = qcs_fusion_plan(gates, ng, n, k, &cp->blocks, &cp->nb, &cp->ordered);
       if (rc == 0) {
-        passes = (qcs_hip_pass *)calloc(1, sizeof(qcs_hip_pass));
-        if (!passes) rc = -1;
+        cp->passes = (qcs_hip_pass *)calloc(1, sizeof(qcs_hip_pass));
+        if (!cp->passes) rc = -1;
         else {
-          np = 1;
-          passes[0].tile_bits = n;
-          passes[0].first_block = 0;
-          passes[0].num_blocks = nb;
-          for (q = 0; q < n; q++) passes[0].bits[q] = q;
+          cp->np = 1;
+          cp->passes[0].tile_bits = n;
+          cp->passes[0].num_blocks = cp->nb;
+          for (q = 0; q < n; q++) cp->passes[0].bits[q] = q;
         }
       }
     } else {
       /* room for a k-qubit block beside the low qubits */
       int low = QCS_BATCH_LOW_BITS_C64;
       if (low > tb - 2) low = tb - 2;
-      rc = qcs_pass_plan(gates, ng, n, tb, low, k, INT_MAX, &passes, &np,
-                         &blocks, &nb, &ordered);
+      rc = qcs_pass_plan(gates, ng, n, tb, low, k, INT_MAX, &cp->passes,
+                         &cp->np, &cp->blocks, &cp->nb, &cp->ordered);
     }
-    gates = ordered;
   } else {
     /* exact gates (CNOT, X, Z, SWAP, ...) move up to their qubits' previous
        gate, which gives the same amplitudes and far fewer passes (the
        layered circuit at 20 qubits: 2.05 instead of 5.5 per layer) */
-    rc = qcs_gate_hoist_exact(gates, ng, n, &ordered);
-    if (rc == 0) {
-      gates = ordered;
-      rc = qcs_gate_pass_plan(gates, ng, n, tb, QCS_BATCH_LOW_BITS_C128,
-                              &passes, &np);
+    rc = qcs_gate_hoist_exact(gates, ng, n, &cp->ordered);
+    if (rc == 0)
+      rc = qcs_gate_pass_plan(cp->ordered, ng, n, tb, QCS_BATCH_LOW_BITS_C128,
+                              &cp->passes, &cp->np);
+  }
+  if (rc == 0 && cp->np == 0) {
+    /* no gates: one empty pass still prepares and samples the state */
+    free(cp->passes);
+    cp->passes = (qcs_hip_pass *)calloc(1, sizeof(qcs_hip_pass));
+    if (!cp->passes) rc = -1;
+    else {
+      cp->np = 1;
+      cp->passes[0].tile_bits = tb;
+      for (q = 0; q < tb; q++) cp->passes[0].bits[q] = q;
     }
   }
-  if (rc != 0) goto out;
-  rc = -1;
-  if (np == 0) { /* no gates: one empty pass still prepares and samples */
-    free(passes);
-    passes = (qcs_hip_pass *)calloc(1, sizeof(qcs_hip_pass));
-    if (!passes) goto out;
-    np = 1;
-    passes[0].tile_bits = tb;
-    for (q = 0; q < tb; q++) passes[0].bits[q] = q;
+  cp->rc = rc;
+}
+
+/* QCS_BATCH_PROFILE=1 prints the host phases of every run to stderr */
+static double now_ms(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec * 1e3 + ts.tv_nsec * 1e-6;
+}
+
+static int profiling(void) {
+  const char *e = getenv("QCS_BATCH_PROFILE");
+  return e && e[0] && e[0] != '0';
+}
+
+static void free_plan(circuit_plan *cp) {
+  free(cp->passes);
+  free(cp->blocks);
+  free(cp->ordered);
+}
+
+/* circuits are planned on up to QCS_BATCH_MAX_THREADS host threads
+ * (QCS_HOST_THREADS overrides the online core count) */
+#define QCS_BATCH_MAX_THREADS 64
+
+typedef struct {
+  const t_q_batch *b;
+  t_q_circuit **circuits;
+  circuit_plan *plans;
+  int count, next;
+  pthread_mutex_t lock;
+} plan_job;
+
+static void *plan_worker(void *arg) {
+  plan_job *job = (plan_job *)arg;
+  for (;;) {
+    int i;
+    pthread_mutex_lock(&job->lock);
+    i = job->next++;
+    pthread_mutex_unlock(&job->lock);
+    if (i >= job->count) break;
+    plan_circuit(job->b, job->circuits[i], &job->plans[i]);
   }
-  if (reserve((void **)&P->pass, &P->cap_p, P->np + np, sizeof(qcs_hip_pass)) ||
-      reserve((void **)&P->block, &P->cap_b, P->nb + nb, sizeof(qcs_hip_block)) ||
-      reserve((void **)&P->gate, &P->cap_g, P->ng + ng, sizeof(qcs_hip_gate)))
-    goto out;
-  memset(&P->circ[i], 0, sizeof(P->circ[i]));
-  P->circ[i].num_qubits = n;
-  P->circ[i].precision = fused ? QCS_HIP_C64 : QCS_HIP_C128;
-  P->circ[i].tile_bits = tb;
-  P->circ[i].first_pass = P->np;
-  P->circ[i].num_passes = np;
-  for (j = 0; j < np; j++) {
-    qcs_hip_pass *d = &P->pass[P->np + j];
-    *d = passes[j];
-    d->first_block += fused ? P->nb : P->ng;
-  }
-  for (j = 0; j < nb; j++) {
-    qcs_hip_block *d = &P->block[P->nb + j];
-    *d = blocks[j];
-    d->first_gate += P->ng;
-  }
-  if (ng > 0) memcpy(&P->gate[P->ng], gates, (size_t)ng * sizeof(qcs_hip_gate));
-  P->np += np;
-  P->nb += nb;
-  P->ng += ng;
-  rc = 0;
-out:
-  free(passes);
-  free(blocks);
-  free(ordered);
-  return rc;
+  return NULL;
+}
+
+static int host_threads(int count) {
+  long t = sysconf(_SC_NPROCESSORS_ONLN);
+  const char *env = getenv("QCS_HOST_THREADS");
+  if (env && atoi(env) > 0) t = atoi(env);
+  if (t < 1) t = 1;
+  if (t > QCS_BATCH_MAX_THREADS) t = QCS_BATCH_MAX_THREADS;
+  if (t > count / 16 + 1) t = count / 16 + 1;
+  return (int)t;
+}
+
+static void plan_all(const t_q_batch *b, t_q_circuit **circuits, int count,
+                     circuit_plan *plans) {
+  plan_job job;
+  pthread_t th[QCS_BATCH_MAX_THREADS];
+  int nt = host_threads(count), started = 0, i;
+  job.b = b;
+  job.circuits = circuits;
+  job.plans = plans;
+  job.count = count;
+  job.next = 0;
+  pthread_mutex_init(&job.lock, NULL);
+  for (i = 1; i < nt; i++)
+    if (pthread_create(&th[started], NULL, plan_worker, &job) == 0) started++;
+  plan_worker(&job); /* the calling thread works too */
+  for (i = 0; i < started; i++) pthread_join(th[i], NULL);
+  pthread_mutex_destroy(&job.lock);
 }
 
 int qc_batch_sample(t_q_batch *batch, t_q_circuit **circuits, int count,
                     int shots, long *indices) {
-  batch_plan P;
+  circuit_plan *plans = NULL;
+  qcs_hip_batch_circuit *circ = NULL;
+  qcs_hip_pass *pass = NULL;
+  qcs_hip_block *block = NULL;
+  qcs_hip_gate *gate = NULL;
   double *u = NULL;
-  long total, s;
-  int i, rc = QCS_HIP_ENOMEM;
+  long total, s, np = 0, nb = 0, ng = 0;
+  int i, j, rc = QCS_HIP_ENOMEM;
+  double t0, t1, t2, t3;
   if (!batch || count < 0 || shots < 0 || (count > 0 && !circuits) ||
       (count > 0 && shots > 0 && !indices))
     return QCS_HIP_EINVAL;
@@ -222,26 +251,77 @@ int qc_batch_sample(t_q_batch *batch, t_q_circuit **circuits, int count,
       return QCS_HIP_EINVAL;
     }
   if (count == 0) return 0;
-  memset(&P, 0, sizeof(P));
-  P.circ = (qcs_hip_batch_circuit *)calloc((size_t)count, sizeof(qcs_hip_batch_circuit));
+  plans = (circuit_plan *)calloc((size_t)count, sizeof(circuit_plan));
+  circ = (qcs_hip_batch_circuit *)calloc((size_t)count, sizeof(qcs_hip_batch_circuit));
   total = (long)count * (long)shots;
   if (total > 0) u = (double *)malloc((size_t)total * sizeof(double));
-  if (!P.circ || (total > 0 && !u)) goto out;
-  for (i = 0; i < count; i++)
-    if (append_circuit(&P, batch, circuits[i], i) != 0) {
+  if (!plans || !circ || (total > 0 && !u)) goto out;
+  t0 = now_ms();
+  plan_all(batch, circuits, count, plans);
+  t1 = now_ms();
+  for (i = 0; i < count; i++) {
+    if (plans[i].rc != 0) {
       fprintf(stderr, "Error: planning batch circuit %d failed.\n", i);
       goto out;
     }
+    np += plans[i].np;
+    nb += plans[i].nb;
+    ng += circuits[i]->pending_size;
+  }
+  if (np > INT_MAX || nb > INT_MAX || ng > INT_MAX) {
+    rc = QCS_HIP_EINVAL;
+    goto out;
+  }
+  pass = (qcs_hip_pass *)malloc((size_t)np * sizeof(qcs_hip_pass));
+  block = (qcs_hip_block *)malloc((size_t)(nb > 0 ? nb : 1) * sizeof(qcs_hip_block));
+  gate = (qcs_hip_gate *)malloc((size_t)(ng > 0 ? ng : 1) * sizeof(qcs_hip_gate));
+  if (!pass || !block || !gate) goto out;
+  /* concatenate: pass / block / gate offsets of circuit i */
+  np = nb = ng = 0;
+  for (i = 0; i < count; i++) {
+    const circuit_plan *cp = &plans[i];
+    const t_q_circuit *c = circuits[i];
+    const int fused = (c->options & QCS_FUSED_FP32) != 0;
+    const int cng = c->pending_size;
+    circ[i].num_qubits = c->num_qubits;
+    circ[i].precision = fused ? QCS_HIP_C64 : QCS_HIP_C128;
+    circ[i].tile_bits = cp->tb;
+    circ[i].first_pass = (int)np;
+    circ[i].num_passes = cp->np;
+    for (j = 0; j < cp->np; j++) {
+      pass[np + j] = cp->passes[j];
+      pass[np + j].first_block += (int)(fused ? nb : ng);
+    }
+    for (j = 0; j < cp->nb; j++) {
+      block[nb + j] = cp->blocks[j];
+      block[nb + j].first_gate += (int)ng;
+    }
+    if (cng > 0)
+      memcpy(&gate[ng], cp->ordered ? cp->ordered : c->pending,
+             (size_t)cng * sizeof(qcs_hip_gate));
+    np += cp->np;
+    nb += cp->nb;
+    ng += cng;
+  }
+  t2 = now_ms();
   for (s = 0; s < total; s++) u[s] = rand() / (double)RAND_MAX;
-  rc = qcs_hip_batch_run(batch->dev, P.circ, count, P.pass, P.np, P.block, P.nb,
-                         P.gate, P.ng, shots, u, indices);
+  t3 = now_ms();
+  rc = qcs_hip_batch_run(batch->dev, circ, count, pass, (int)np, block, (int)nb,
+                         gate, (int)ng, shots, u, indices);
+  if (profiling())
+    fprintf(stderr, "qc_batch_sample: plan %.1f ms (%d threads), concatenate %.1f ms, "
+                    "uniforms %.1f ms, run %.1f ms\n", t1 - t0, host_threads(count),
+            t2 - t1, t3 - t2, now_ms() - t3);
   if (rc != 0)
     fprintf(stderr, "Error: batch run failed: %s\n", qcs_hip_last_error());
 out:
-  free(P.circ);
-  free(P.pass);
-  free(P.block);
-  free(P.gate);
+  if (plans)
+    for (i = 0; i < count; i++) free_plan(&plans[i]);
+  free(plans);
+  free(circ);
+  free(pass);
+  free(block);
+  free(gate);
   free(u);
   return rc;
 }
