@@ -19,7 +19,12 @@ namespace qcs {
 // XCD-aware work-group order (device.h) keep the in-place sweeps near the
 // copy bandwidth: 5.6-6.6 TB/s for a complex128 1-qubit gate at 30 qubits,
 // against 5.1-5.9 TB/s for two pairs per thread with cached accesses.
+// A dispatch holds fewer than 2^32 work-items, so the grid is capped at
+// kMaxGrid work-groups (a multiple of 8) and each work-group walks the
+// virtual blocks vb = blockIdx.x + i * gridDim.x; the XCD order applies to
+// the virtual block index, which shares blockIdx.x's XCD.
 constexpr int kUnroll = 1;
+constexpr uint64_t kMaxGrid = uint64_t(1) << 20;
 
 // ---- 1-qubit gates --------------------------------------------------------
 
@@ -30,8 +35,9 @@ __global__ __launch_bounds__(kThreads) void k_1q_dense(
     uint64_t npairs, int t, Mat2<T> m) {
   typedef typename Amp<T>::type A;
   const uint64_t tb = uint64_t(1) << t;
-  const uint64_t base =
-      xcd_order(blockIdx.x, gridDim.x) * (kThreads * kUnroll) + threadIdx.x;
+  const uint64_t nvb = ceil_div(npairs, uint64_t(kThreads) * kUnroll);
+  for (uint64_t vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
+  const uint64_t base = xcd_order(vb, nvb) * (kThreads * kUnroll) + threadIdx.x;
   A v0[kUnroll], v1[kUnroll];
   uint64_t i0[kUnroll];
 #pragma unroll
@@ -55,6 +61,7 @@ __global__ __launch_bounds__(kThreads) void k_1q_dense(
     st_stream(out + i0[u], o0);
     st_stream(out + (i0[u] | tb), o1);
   }
+  }
 }
 
 // diag(1, m11) on target t (Z, P): only the bit-t=1 half changes.
@@ -63,8 +70,9 @@ __global__ __launch_bounds__(kThreads) void k_1q_phase(
     typename Amp<T>::type *a, uint64_t nhalf, int t, T mr, T mi) {
   typedef typename Amp<T>::type A;
   const uint64_t tb = uint64_t(1) << t;
-  const uint64_t base =
-      xcd_order(blockIdx.x, gridDim.x) * (kThreads * kUnroll) + threadIdx.x;
+  const uint64_t nvb = ceil_div(nhalf, uint64_t(kThreads) * kUnroll);
+  for (uint64_t vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
+  const uint64_t base = xcd_order(vb, nvb) * (kThreads * kUnroll) + threadIdx.x;
   A v[kUnroll];
   uint64_t idx[kUnroll];
 #pragma unroll
@@ -81,6 +89,7 @@ __global__ __launch_bounds__(kThreads) void k_1q_phase(
     cmul(mr, mi, v[u].x, v[u].y, o.x, o.y);
     st_stream(a + idx[u], o);
   }
+  }
 }
 
 // ---- controlled gates (textbook) --------------------------------------------
@@ -93,8 +102,9 @@ __global__ __launch_bounds__(kThreads) void k_cu(typename Amp<T>::type *a,
                                                  uint64_t tb, Mat2<T> m) {
   // KIND 0: dense 2x2 on (i0, i1); 1: swap (CNOT); 2: phase on i1 only (CP)
   typedef typename Amp<T>::type A;
-  const uint64_t base =
-      xcd_order(blockIdx.x, gridDim.x) * (kThreads * kUnroll) + threadIdx.x;
+  const uint64_t nvb = ceil_div(nquarter, uint64_t(kThreads) * kUnroll);
+  for (uint64_t vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
+  const uint64_t base = xcd_order(vb, nvb) * (kThreads * kUnroll) + threadIdx.x;
   A v0[kUnroll], v1[kUnroll];
   uint64_t i0[kUnroll];
 #pragma unroll
@@ -127,6 +137,7 @@ __global__ __launch_bounds__(kThreads) void k_cu(typename Amp<T>::type *a,
       st_stream(a + (i0[u] | tb), o);
     }
   }
+  }
 }
 
 // Compat controlled gate, out of place over all N (src/q_gates.c:277-292 as
@@ -137,7 +148,9 @@ __global__ __launch_bounds__(kThreads) void k_cu_compat(
     const typename Amp<T>::type *in, typename Amp<T>::type *out, uint64_t n,
     uint64_t cb, uint64_t tb, Mat2<T> m) {
   typedef typename Amp<T>::type A;
-  const uint64_t base = uint64_t(blockIdx.x) * (kThreads * kUnroll) + threadIdx.x;
+  const uint64_t nvb = ceil_div(n, uint64_t(kThreads) * kUnroll);
+  for (uint64_t vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
+  const uint64_t base = vb * (kThreads * kUnroll) + threadIdx.x;
 #pragma unroll
   for (int u = 0; u < kUnroll; ++u) {
     const uint64_t i = base + uint64_t(u) * kThreads;
@@ -152,6 +165,7 @@ __global__ __launch_bounds__(kThreads) void k_cu_compat(
       out[i] = v;
     }
   }
+  }
 }
 
 // SWAP of qubits a < b: exchange amplitudes with (bit a, bit b) = (1,0), (0,1)
@@ -161,8 +175,9 @@ __global__ __launch_bounds__(kThreads) void k_swapq(typename Amp<T>::type *a,
                                                     int hi) {
   typedef typename Amp<T>::type A;
   const uint64_t bl = uint64_t(1) << lo, bh = uint64_t(1) << hi;
-  const uint64_t base =
-      xcd_order(blockIdx.x, gridDim.x) * (kThreads * kUnroll) + threadIdx.x;
+  const uint64_t nvb = ceil_div(nquarter, uint64_t(kThreads) * kUnroll);
+  for (uint64_t vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
+  const uint64_t base = xcd_order(vb, nvb) * (kThreads * kUnroll) + threadIdx.x;
 #pragma unroll
   for (int u = 0; u < kUnroll; ++u) {
     const uint64_t q = base + uint64_t(u) * kThreads;
@@ -172,6 +187,7 @@ __global__ __launch_bounds__(kThreads) void k_swapq(typename Amp<T>::type *a,
     const A y = ld_stream(a + (i | bh));
     st_stream(a + (i | bl), y);
     st_stream(a + (i | bh), x);
+  }
   }
 }
 
@@ -212,7 +228,8 @@ bool is_x(const double m[8]) {
 }
 
 static unsigned grid_for(uint64_t items) {
-  return unsigned(ceil_div(items, uint64_t(kThreads) * kUnroll));
+  const uint64_t g = ceil_div(items, uint64_t(kThreads) * kUnroll);
+  return unsigned(g < kMaxGrid ? g : kMaxGrid);
 }
 
 template <typename T>

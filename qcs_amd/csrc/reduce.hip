@@ -5,6 +5,7 @@
 // Every reduction runs on a fixed grid of <= kRedBlocks work-groups that
 // write per-block partials; one work-group then folds the partials in a fixed
 // order, so results are bit-reproducible run to run.
+#include <algorithm>
 #include <cmath>
 
 #include "device.h"
@@ -677,11 +678,16 @@ static int sample_impl(qcs_hip_state *s, long shots, const double *u,
                      s->chunk_tot, s->chunk_inc, nchunks);
   if ((rc = launch_check("k_chunk_scan"))) return rc;
   timing_begin(s, QCS_HIP_K_OTHER, double(shots) * len * sizeof(A), 0.0, &tl);
-  hipLaunchKernelGGL(k_sample<T>, dim3(unsigned(ceil_div(uint64_t(shots), kThreads / 64))),
-                     dim3(kThreads), 0, s->stream, a, nchunks, len,
-                     s->chunk_tot, s->chunk_inc, s->u_dev, uint64_t(shots),
-                     s->idx_dev);
-  if ((rc = launch_check("k_sample"))) return rc;
+  // one wave per shot; at most 2^24 shots per dispatch keeps the
+  // work-item count below 2^32
+  for (uint64_t s0 = 0; s0 < uint64_t(shots); s0 += uint64_t(1) << 24) {
+    const uint64_t ns = std::min<uint64_t>(uint64_t(shots) - s0, uint64_t(1) << 24);
+    hipLaunchKernelGGL(k_sample<T>, dim3(unsigned(ceil_div(ns, kThreads / 64))),
+                       dim3(kThreads), 0, s->stream, a, nchunks, len,
+                       s->chunk_tot, s->chunk_inc, s->u_dev + s0, ns,
+                       s->idx_dev + s0);
+    if ((rc = launch_check("k_sample"))) return rc;
+  }
   if ((rc = timing_end(s, &tl))) return rc;
   QCS_HIP_TRY(hipMemcpyAsync(out, s->idx_dev, size_t(shots) * sizeof(long),
                              hipMemcpyDeviceToHost, s->stream));
