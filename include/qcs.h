@@ -149,8 +149,9 @@ void *qc_get_device_state(t_q_circuit *circuit);
 /* Batched execution of QCS_DEFERRED circuits: every circuit starts from
  * |0...0>, runs its recorded gates and is sampled, all in one persistent
  * device launch (small states stay in on-chip memory).  A complex128 circuit
- * gives the amplitudes and samples of qc_create + the same gates +
- * qc_sample, bit for bit; a QCS_FUSED_FP32 one those of the fused form
+ * gives the amplitudes of qc_create + the same gates (equal values; only
+ * the sign of a zero may differ) and the samples of qc_sample; a
+ * QCS_FUSED_FP32 one those of the fused form
  * within its tolerance.  Not thread-safe, like the rest of the library. */
 typedef struct s_q_batch t_q_batch;
 #define QCS_BATCH_GRAPH 0x1       /* capture the launch sequence as a graph
@@ -187,5 +188,21 @@ int qc_batch_set_plan(t_q_batch *batch, int fused, int num_qubits,
                       int tile_bits, int fuse_k);
 /* threads per work-group of the batch kernel: 256 or 512 (default). */
 int qc_batch_set_threads(t_q_batch *batch, int threads);
+
+/* Autotuning, cached per device in a text file ($QCS_TUNE_CACHE, a path or
+ * "off"; else $XDG_CACHE_HOME/qcs_amd/tune.txt or
+ * ~/.cache/qcs_amd/tune.txt).
+ * qc_autotune: fused circuit -- times the fused-pass kernel variants of
+ * every width k on the circuit's state (identity blocks, the state is left
+ * as it is), keeps the fastest per (n, k) and picks the fusion width for
+ * the recorded gates (blocks x time per pass).  Returns that width or a
+ * negative QCS_HIP_E* code. */
+int qc_autotune(t_q_circuit *circuit);
+/* qc_batch_autotune: for every (precision, n) of the given deferred
+ * circuits whose state is streamed through on-chip tiles, times candidate
+ * tile sizes (and fusion widths) on a sample of them with zero shots (no
+ * rand() draws), then the work-group size; sets the batch's plan.  Returns
+ * 0 or a negative QCS_HIP_E* code. */
+int qc_batch_autotune(t_q_batch *batch, t_q_circuit **circuits, int count);
 
 #endif

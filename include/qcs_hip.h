@@ -50,6 +50,9 @@ const char *qcs_hip_last_error(void);
 int qcs_hip_device_available(void);
 /* hipDeviceSynchronize on the current device */
 int qcs_hip_device_sync(void);
+/* "<gcn arch>:<compute units>" of the current device (e.g. "gfx950:256"),
+ * the key of cached autotuning results; no reference counterpart */
+int qcs_hip_device_name(char *buf, int len);
 
 /* ---- state lifetime ---------------------------------------------------
  * Replaces q_state_init / q_state_free (src/q_state.c:32-115): allocates
@@ -245,9 +248,10 @@ int qcs_hip_timing_read(qcs_hip_state *s, int kernel_class, double *total_ms,
  * keeps the state in LDS when it fits (n <= tile_bits) and otherwise
  * streams it through LDS tile by tile, one pass per entry of its pass list.
  *   QCS_HIP_C128: the gates are applied one by one with the arithmetic of
- *     qcs_hip_apply_1q / qcs_hip_apply_cu / qcs_hip_apply_swap, so the final
- *     amplitudes and samples equal those of a qcs_hip_state of its own bit
- *     for bit.  A pass's first_block / num_blocks count gates; a pass is a
+ *     qcs_hip_apply_1q / qcs_hip_apply_cu / qcs_hip_apply_swap, leaving out
+ *     only products by exact-zero matrix entries, so the final amplitudes
+ *     equal those of a qcs_hip_state of its own (equal values; a zero may
+ *     differ in sign) and so do the samples.  A pass's first_block / num_blocks count gates; a pass is a
  *     run of consecutive gates whose qubits lie in the pass's bits.
  *   QCS_HIP_C64: fused precision form.  A pass's blocks (composed on the
  *     device in FP64 like qcs_hip_apply_fused) are applied on the FP32

@@ -213,3 +213,8 @@ class Batch:
         rc = self.lib.qc_batch_set_threads(self._b, threads)
         if rc != 0:
             raise ValueError(f"qc_batch_set_threads rejected ({rc})")
+
+    def autotune(self, circuits_):
+        rc = self.lib.qc_batch_autotune(self._b, self._handles(circuits_), len(circuits_))
+        if rc != 0:
+            raise RuntimeError(f"qc_batch_autotune failed ({rc})")

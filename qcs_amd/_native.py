@@ -72,6 +72,7 @@ _HIP_PROTOS = {
     "qcs_hip_last_error": (ctypes.c_char_p, []),
     "qcs_hip_device_available": (c_int, []),
     "qcs_hip_device_sync": (c_int, []),
+    "qcs_hip_device_name": (c_int, [ctypes.c_char_p, c_int]),
     "qcs_hip_state_create": (c_int, [P(c_void_p), c_int, c_int, c_int]),
     "qcs_hip_state_destroy": (None, [c_void_p]),
     "qcs_hip_state_num_qubits": (c_int, [c_void_p]),
@@ -180,6 +181,8 @@ _QCS_PROTOS = {
     "qc_batch_last_timing": (c_int, [c_void_p, P(c_double), P(c_double)]),
     "qc_batch_set_plan": (c_int, [c_void_p, c_int, c_int, c_int, c_int]),
     "qc_batch_set_threads": (c_int, [c_void_p, c_int]),
+    "qc_autotune": (c_int, [_Q]),
+    "qc_batch_autotune": (c_int, [c_void_p, P(_Q), c_int]),
 }
 
 _hip = None

@@ -220,6 +220,20 @@ int qcs_hip_device_sync(void) {
   return 0;
 }
 
+int qcs_hip_device_name(char *buf, int len) {
+  if (!buf || len < 1) return fail(QCS_HIP_EINVAL, "bad buffer");
+  int dev = 0;
+  QCS_HIP_TRY(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  QCS_HIP_TRY(hipGetDeviceProperties(&prop, dev));
+  char arch[64];
+  std::snprintf(arch, sizeof(arch), "%s", prop.gcnArchName);
+  for (char *c = arch; *c; ++c)
+    if (*c == ':' || *c == ' ') *c = '\0';  // drop target features
+  std::snprintf(buf, size_t(len), "%s:%d", arch, prop.multiProcessorCount);
+  return 0;
+}
+
 int qcs_hip_state_create(qcs_hip_state **out, int num_qubits, int precision,
                          int flags) {
   if (!out) return fail(QCS_HIP_EINVAL, "null output handle");

@@ -155,7 +155,7 @@ static int use_tiles(const t_q_circuit *c) {
 }
 
 static int fuse_k(const t_q_circuit *c) {
-  int k = (c->options >> 8) & 0xff;
+  int k = c->tuned_k > 0 ? c->tuned_k : (c->options >> 8) & 0xff;
   if (k < 1 || k > QCS_HIP_MAX_FUSE) k = use_tiles(c) ? QCS_HIP_TILE_KMAX : QCS_HIP_MAX_FUSE;
   if (use_tiles(c) && k > QCS_HIP_TILE_KMAX) k = QCS_HIP_TILE_KMAX;
   if (k > c->num_qubits) k = c->num_qubits;

@@ -24,6 +24,7 @@ struct t_q_circuit {
   int pending_capacity;
   long fused_blocks;
   long fused_passes;
+  int tuned_k; /* fusion width chosen by qc_autotune (0: none) */
 };
 
 /* Greedy fusion planner (fusion.c).
