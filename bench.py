@@ -16,8 +16,9 @@ Beside it, in the same process:
     matrix-core kernel from HIP events recorded on the state's stream inside
     the timed steps; algorithmic bytes per launch = 2 * 2^30 * 8;
   * the CPU baseline: the reference's own sequential library (oracle/_ref,
-    built from its sources) on the 20-qubit config-1 circuit, first gates
-    only (a bounded ~15 s sample), 1 core.
+    built from its sources) on the 20-qubit depth-10 circuit, first gates
+    only (a bounded sample), 1 core; beside it the reference's OpenMP build
+    and the oracle's race-fixed OpenMP loop on the box's cores.
 
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -59,6 +60,8 @@ def parse():
                         "4096 independent 10-20 qubit circuits in one persistent launch")
     p.add_argument("--batch-count", type=int, default=4096)
     p.add_argument("--batch-threads", type=int, default=512)
+    p.add_argument("--no-tune", action="store_true",
+                   help="skip the autotuner (default plans and kernel variants)")
     p.add_argument("--copy-probe", type=float, default=0.0, metavar="GIB",
                    help="only run a plain device copy of GIB GiB (float32, out = in * 1) "
                         "three times: the known-size kernel that scales HBM byte "
@@ -101,7 +104,8 @@ def cpu_model():
 
 
 def cpu_baseline(seconds):
-    """Reference sequential library on the 20-qubit config-1 circuit."""
+    """Reference sequential library on the 20-qubit depth-10 circuit, then its
+    OpenMP build and the race-fixed OpenMP restatement."""
     from qcs_amd import circuits as C
     n = 20
     ops = C.layered(n, 10, 0)
@@ -123,11 +127,49 @@ def cpu_baseline(seconds):
         close = st.close
     done = 0
     t0 = time.perf_counter()
-    while done < len(ops) and time.perf_counter() - t0 < seconds:
+    while done < len(ops) and time.perf_counter() - t0 < 0.5 * seconds:
         apply_one(done)
         done += 1
     dt = time.perf_counter() - t0
     close()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    omp = {}
+    try:
+        # the reference's OpenMP build, as written (its shared loop index races)
+        from oracle.oracle import RefLib
+        refo = RefLib("libqcsref_omp.so")
+        co = refo.create(n)
+        k = 0
+        t1 = time.perf_counter()
+        while k < len(ops) and time.perf_counter() - t1 < 0.25 * seconds:
+            refo.apply(co, _slice(ops, k))
+            k += 1
+        d1 = time.perf_counter() - t1
+        refo.L.qc_destroy(co)
+        omp["openmp_reference"] = {
+            "value": k * float(1 << n) / d1, "unit": "amplitude updates/s",
+            "cores": threads, "kind": "reference",
+            "sample": f"first {k} gates, {d1:.1f} s, reference OpenMP build (racy as written)"}
+    except (FileNotFoundError, OSError):
+        pass
+    # the oracle's OpenMP restatement of the 1-qubit loop with the race fixed
+    from oracle.oracle import OracleState, gate as ogate
+    st = OracleState(n)
+    k = 0
+    t2 = time.perf_counter()
+    one_q = [i for i in range(len(ops)) if ops.op[i] < C.CNOT]
+    while k < len(one_q) and time.perf_counter() - t2 < 0.25 * seconds:
+        i = one_q[k]
+        m = ogate(ops.op[i], ops.par[i])
+        st.L.oq_apply_1q_omp(st.s, int(ops.tq[i]), m.ctypes.data_as(
+            __import__("ctypes").POINTER(__import__("ctypes").c_double)), 0)
+        k += 1
+    d2 = time.perf_counter() - t2
+    st.close()
+    omp["openmp_race_fixed"] = {
+        "value": k * float(1 << n) / d2, "unit": "amplitude updates/s",
+        "cores": threads, "kind": "port",
+        "sample": f"first {k} one-qubit gates, {d2:.1f} s, OpenMP restatement with a private index"}
     return {
         "value": done * float(1 << n) / dt,
         "unit": "amplitude updates/s",
@@ -138,6 +180,7 @@ def cpu_baseline(seconds):
         "gates_per_s": done / dt,
         "cpu": cpu_model(),
         "nproc": os.cpu_count(),
+        **omp,
     }
 
 
@@ -187,6 +230,8 @@ def bench_batch(args):
     # one run without a graph times the batch kernel alone (HIP events)
     b0 = qcs_amd.Batch()
     b0.set_threads(args.batch_threads)
+    if not args.no_tune:
+        b0.autotune(circs)
     t0 = time.perf_counter()
     b0.sample(circs, shots)
     first_s = time.perf_counter() - t0
@@ -196,6 +241,11 @@ def bench_batch(args):
           f"kernel {kernel_ms:.1f} ms", file=sys.stderr, flush=True)
     b = qcs_amd.Batch(qcs_amd.BATCH_GRAPH)
     b.set_threads(args.batch_threads)
+    if not args.no_tune:
+        t0 = time.perf_counter()
+        b.autotune(circs)
+        print(f"batch: autotuned in {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+              flush=True)
     for _ in range(max(args.warmup, 1)):
         b.sample(circs, shots)
     hip.qcs_hip_device_sync()
@@ -276,6 +326,11 @@ def main():
     fused = qcs_amd.Circuit(n, opts)
     hip.qcs_hip_set_tuning(fused.device_state, qcs_amd.TUNE_TILE_THREADS, args.tile_threads)
     fuse_k = args.fuse_k or (4 if args.mode == "tiles" else 5)
+    if not args.no_tune and not args.fuse_k:
+        # kernel variant per (n, k) and fusion width for this circuit
+        fused.apply(ops)
+        fuse_k = fused.autotune()
+        fused.flush()
 
     def step():
         fused.set_basis_state(0)
@@ -329,6 +384,7 @@ def main():
             "depth": depth,
             "gates": G,
             "fusion_k": fuse_k,
+            "autotuned": not args.no_tune and not args.fuse_k,
             "fusion_mode": args.mode,
             "tile_threads": args.tile_threads,
             "fused_blocks": nblocks,
