@@ -793,33 +793,79 @@ int qc_get_num_gates(t_q_circuit *circuit) { return circuit->num_gates; }
  * adjacent identical CNOT pairs from the history, restarting the scan after
  * each removal.  Like the reference it edits the log only; the state is not
  * touched. */
-void qc_optimize(t_q_circuit *circuit) {
-  int i = 0, j;
-  if (!circuit) return;
-  while (i < circuit->history_size - 1) {
-    const char *a = circuit->gate_history[i];
-    const char *b = circuit->gate_history[i + 1];
-    int t1 = circuit->target_qubits[i], t2 = circuit->target_qubits[i + 1];
-    int c1 = circuit->control_qubits[i], c2 = circuit->control_qubits[i + 1];
-    int one = (t1 == t2) && strcmp(a, b) == 0 &&
-              (strcmp(a, "H") == 0 || strcmp(a, "X") == 0 ||
-               strcmp(a, "Y") == 0 || strcmp(a, "Z") == 0);
-    int cx = strcmp(a, "CNOT") == 0 && strcmp(b, "CNOT") == 0 && t1 == t2 &&
-             c1 == c2;
-    if (one || cx) {
-      free(circuit->gate_history[i]);
-      free(circuit->gate_history[i + 1]);
-      for (j = i; j < circuit->history_size - 2; j++) {
-        circuit->gate_history[j] = circuit->gate_history[j + 2];
-        circuit->target_qubits[j] = circuit->target_qubits[j + 2];
-        circuit->control_qubits[j] = circuit->control_qubits[j + 2];
-        circuit->parameters[j] = circuit->parameters[j + 2];
-      }
-      circuit->history_size -= 2;
-      circuit->num_gates -= 2;
-      i = 0;
-    } else {
-      i++;
-    }
+/* history entry that qc_optimize cancels against an identical neighbour:
+ * H, X, Y, Z and CNOT (src/qcs.c:671-678) */
+static int history_involution(const t_q_circuit *c, int i) {
+  const char *g = c->gate_history[i];
+  return strcmp(g, "H") == 0 || strcmp(g, "X") == 0 || strcmp(g, "Y") == 0 ||
+         strcmp(g, "Z") == 0 || strcmp(g, "CNOT") == 0;
+}
+
+static int history_same(const t_q_circuit *c, int i, int j) {
+  return strcmp(c->gate_history[i], c->gate_history[j]) == 0 &&
+         c->target_qubits[i] == c->target_qubits[j] &&
+         (strcmp(c->gate_history[i], "CNOT") != 0 ||
+          c->control_qubits[i] == c->control_qubits[j]);
+}
+
+/* recorded gate qc_optimize may drop in pairs: the same self-inverse
+ * matrices.  A deferred complex128 circuit keeps H pairs, whose product is
+ * not exactly the identity in floating point, so that it still matches the
+ * eager plain form amplitude for amplitude. */
+static int pending_involution(const t_q_circuit *c, const qcs_hip_gate *g) {
+  double m[8];
+  if (g->kind == QCS_HIP_G_CU) {
+    gate_x(m);
+    return memcmp(g->m, m, sizeof(m)) == 0;
   }
+  if (g->kind != QCS_HIP_G_1Q) return 0;
+  gate_x(m);
+  if (memcmp(g->m, m, sizeof(m)) == 0) return 1;
+  gate_y(m);
+  if (memcmp(g->m, m, sizeof(m)) == 0) return 1;
+  gate_z(m);
+  if (memcmp(g->m, m, sizeof(m)) == 0) return 1;
+  gate_h(m);
+  return is_fused(c) && memcmp(g->m, m, sizeof(m)) == 0;
+}
+
+/* src/qcs.c:650-698: drop adjacent identical H/X/Y/Z/CNOT pairs until none
+ * is left.  Pairs of self-inverse entries reduce to a unique normal form,
+ * so one stack pass gives the reference's result in linear time.  Gates
+ * recorded and not yet applied (fused and deferred forms) are reduced the
+ * same way, so the cancelled pairs are never computed. */
+void qc_optimize(t_q_circuit *circuit) {
+  int i, top = 0;
+  if (!circuit) return;
+  for (i = 0; i < circuit->history_size; i++) {
+    if (top > 0 && history_involution(circuit, i) &&
+        history_same(circuit, top - 1, i)) {
+      free(circuit->gate_history[top - 1]);
+      free(circuit->gate_history[i]);
+      top--;
+      circuit->num_gates -= 2;
+      continue;
+    }
+    circuit->gate_history[top] = circuit->gate_history[i];
+    circuit->target_qubits[top] = circuit->target_qubits[i];
+    circuit->control_qubits[top] = circuit->control_qubits[i];
+    circuit->parameters[top] = circuit->parameters[i];
+    top++;
+  }
+  circuit->history_size = top;
+  top = 0;
+  for (i = 0; i < circuit->pending_size; i++) {
+    const qcs_hip_gate *g = &circuit->pending[i];
+    if (top > 0 && pending_involution(circuit, g)) {
+      const qcs_hip_gate *p = &circuit->pending[top - 1];
+      if (p->kind == g->kind && p->target == g->target &&
+          (g->kind == QCS_HIP_G_1Q || p->control == g->control) &&
+          memcmp(p->m, g->m, sizeof(g->m)) == 0) {
+        top--;
+        continue;
+      }
+    }
+    circuit->pending[top++] = *g;
+  }
+  circuit->pending_size = top;
 }

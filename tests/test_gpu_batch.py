@@ -182,3 +182,36 @@ def test_rejects_non_deferred(qlib, cl):
         b.sample([c], 4)
     c.close()
     b.close()
+
+
+def test_optimize_cancels_recorded_pairs(qlib, cl):
+    # X X, Y Y, Z Z, CNOT CNOT pairs vanish from the recorded gates; the
+    # batch still equals the eager plain form, which computes every gate
+    n = 9
+    ops = C.Ops()
+    for q in range(n):
+        ops.add(C.H, q).add(C.X, q).add(C.X, q).add(C.RY, q, -1, 0.3 + q)
+    ops.add(C.CNOT, 1, 0).add(C.CNOT, 1, 0).add(C.Z, 4).add(C.Z, 4).add(C.Y, 2).add(C.Y, 2)
+    d = qcs_amd.Circuit(n, qcs_amd.DEFERRED)
+    d.apply(ops)
+    d.optimize()
+    assert d.get_num_gates() == len(ops.op) - 2 * (n + 3)
+    b = qcs_amd.Batch(qcs_amd.BATCH_KEEP_STATES)
+    cl.srand(8)
+    idx = b.sample([d], 32)
+    with qcs_amd.Circuit(n) as e:
+        e.apply(ops)
+        cl.srand(8)
+        assert np.array_equal(b.amplitudes(0, n), e.amplitudes())
+        assert np.array_equal(idx[0], e.sample_indices(32))
+    d.close()
+    b.close()
+    # the fused form also drops H H pairs
+    with qcs_amd.Circuit(n, qcs_amd.FUSED_FP32) as f, qcs_amd.Circuit(n) as e:
+        f.apply(ops)
+        f.h(3)
+        f.h(3)
+        f.optimize()
+        e.apply(ops)
+        omf, _ = f.compare_with(e)
+        assert omf <= 1e-6
