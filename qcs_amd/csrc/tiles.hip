@@ -10,13 +10,16 @@
 // HBM.  HBM traffic per pass is one read and one write of the state, however
 // many blocks it holds.
 //
-// LDS layout.  Tile element e (bit j of e = index bit S[j]) lives at float2
-// slot swz(e) = e ^ (e>>5 & 31) ^ (e>>10 & 7).  swz is linear over GF(2), so
-// the slot of (column c, row s) of a block is swz(col) ^ swz(row): the host
-// precomputes the 16 row terms of every block and each B-operand read or
-// result write costs one XOR.  The fold puts index bits 5..12 onto the bank
-// bits, so the 32 columns an MFMA operand reads (the 5 lowest non-target
-// tile bits) hit 32 distinct banks or at worst two per bank.  After the tile
+// LDS layout.  The tile is two planes of 8192 floats, real parts then
+// imaginary parts; tile element e (bit j of e = index bit S[j]) sits at slot
+// swz(e) = e ^ (e>>5 & 31) ^ (e>>10 & 7) of both.  swz is linear over GF(2),
+// so the slot of (column c, row s) of a block is swz(col) ^ swz(row): the
+// host precomputes the 16 row terms of every block and each B-operand read
+// or result write costs one XOR.  The fold puts index bits 5..12 onto the
+// low slot bits, so the 32 columns a half-wave reads (the 5 lowest
+// non-target tile bits) are 32 distinct slots mod 32; in separate planes
+// those are 32 distinct 4-byte banks (interleaved {re, im} would put a
+// half-wave's 32 real parts 8 bytes apart, two per bank).  After the tile
 // come the A operands of every block of the pass, staged once per
 // work-group in lane order; they stay in LDS because on gfx9 every vector
 // memory load retires through one in-order counter, so a global load inside
@@ -58,7 +61,7 @@ struct TileBlock {
   int k;
   int wslot;
   int tpos[kTileKMax];        // tile-bit positions of the targets, ascending
-  int rowslot[kTileRMax];     // byte offset 8*swz(row bits of s) per row s
+  int rowslot[kTileRMax];     // slot swz(row bits of s) of row s
 };
 
 __host__ __device__ __forceinline__ int swz(int e) {
@@ -80,6 +83,68 @@ __device__ __forceinline__ int ins0(int x, int b) {
   return ((x >> b) << (b + 1)) | (x & ((1 << b) - 1));
 }
 
+// One block of a pass on the tile, its width K known at compile time.  The
+// B operand rows and the target positions are uniform (scalar loads); the
+// output rows of a lane depend on its half-wave, so their slots are computed
+// here once per block instead of being read per lane from global memory --
+// a vector load inside the tile loop would wait behind the next tile's
+// prefetch, which shares the in-order vmcnt counter.
+template <int K, int NTH, int TILE>
+__device__ __forceinline__ void tp_block(char *ldsb, const TileBlock *__restrict__ bl,
+                                         const float *wlb) {
+  constexpr int R = 1 << K, NW = NTH / 64, NCOLS = TILE >> K;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int h = lane >> 5, l32 = lane & 31;
+  float *re = reinterpret_cast<float *>(ldsb);
+  float *im = re + TILE;
+  const float *bplane = h ? im : re;  // the lane's part of the B operand
+  float wa[R];
+#pragma unroll
+  for (int s = 0; s < R; ++s) wa[s] = wlb[s * 64 + lane];
+  int tpos[K], rs[R];
+#pragma unroll
+  for (int q = 0; q < K; ++q) tpos[q] = bl->tpos[q];
+#pragma unroll
+  for (int s = 0; s < R; ++s) rs[s] = bl->rowslot[s];
+  // output rows of this lane: accumulators 4q, 4q+1 hold row 4q + 2h,
+  // 4q+2, 4q+3 row 4q + 2h + 1; both candidates are uniform values
+  int os0[4], os1[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int a0 = rs[4 * q < R ? 4 * q : 0], a1 = rs[4 * q + 1 < R ? 4 * q + 1 : 0];
+    const int b0 = rs[4 * q + 2 < R ? 4 * q + 2 : 0], b1 = rs[4 * q + 3 < R ? 4 * q + 3 : 0];
+    os0[q] = __builtin_amdgcn_readfirstlane(a0) ^ (h ? (b0 ^ a0) : 0);
+    os1[q] = __builtin_amdgcn_readfirstlane(a1) ^ (h ? (b1 ^ a1) : 0);
+  }
+  for (int c0 = wave * 32; c0 < NCOLS; c0 += NW * 32) {
+    int col = c0 + l32;
+#pragma unroll
+    for (int q = 0; q < K; ++q) col = ins0(col, tpos[q]);
+    const int cs = swz(col);  // slot of (col, row 0)
+    float bv[R];
+#pragma unroll
+    for (int s = 0; s < R; ++s) bv[s] = bplane[cs ^ rs[s]];
+    floatx16 acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f,
+                    0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < R; ++s)
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(wa[s], bv[s], acc, 0, 0, 0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (4 * q >= R) continue;
+      const int r0 = 4 * q + 2 * h;
+      if (r0 < R) {
+        re[cs ^ os0[q]] = acc[4 * q];
+        im[cs ^ os0[q]] = acc[4 * q + 1];
+      }
+      if (r0 + 1 < R) {
+        re[cs ^ os1[q]] = acc[4 * q + 2];
+        im[cs ^ os1[q]] = acc[4 * q + 3];
+      }
+    }
+  }
+}
+
 // The descriptor pointers are __restrict__ so the compiler may read them
 // through the scalar cache (they are uniform and never written by the
 // kernel); vector loads of them would wait behind the tile prefetch.
@@ -89,7 +154,6 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
     const TileBlock *__restrict__ blocks, const float *__restrict__ wbuf,
     uint64_t ntiles) {
   constexpr int TILE = 1 << TB;
-  constexpr int NW = NTH / 64;
   constexpr int TBITS = __builtin_ctz(NTH) + 1;  // tile bits covered by 2*tid
   constexpr int IT = TILE / (2 * NTH);           // float4 loads per thread
   constexpr int IB = TB - TBITS;                 // tile bits covered by `it`
@@ -100,8 +164,7 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
 
   const TilePass &ps = passes[pass_index];
   const int nb = ps.nblocks;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int h = lane >> 5, l32 = lane & 31;
+  const int tid = threadIdx.x;
 
   // stage the A operands of every block: wl[(b*RM + s)*64 + lane] holds
   // W_b[l32][2s + h] (zero outside the block's 2^(k+1) x 2^(k+1) matrix)
@@ -156,8 +219,11 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
       const int s0 = slot_t ^ swz(it << TBITS);  // slot of e; e+1 is s0 ^ 1
-      *reinterpret_cast<float2 *>(ldsb + 8 * s0) = make_float2(v[it].x, v[it].y);
-      *reinterpret_cast<float2 *>(ldsb + 8 * (s0 ^ 1)) = make_float2(v[it].z, v[it].w);
+      float *re = reinterpret_cast<float *>(ldsb), *im = re + TILE;
+      re[s0] = v[it].x;
+      im[s0] = v[it].y;
+      re[s0 ^ 1] = v[it].z;
+      im[s0 ^ 1] = v[it].w;
     }
     __syncthreads();
     const uint64_t next = tile + G8;
@@ -169,39 +235,13 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
     }
 
     for (int b = 0; b < nb; ++b) {
-      const TileBlock &bl = blocks[ps.first_block + b];
-      const int k = bl.k, R = 1 << k;
-      float wa[RM];
-#pragma unroll
-      for (int s = 0; s < RM; ++s) wa[s] = wl[(b * RM + s) * 64 + lane];
-      const int ncols = TILE >> k;
-      for (int c0 = wave * 32; c0 < ncols; c0 += NW * 32) {
-        int col = c0 + l32;
-#pragma unroll
-        for (int q = 0; q < kTileKMax; ++q)
-          if (q < k) col = ins0(col, bl.tpos[q]);
-        const int cs = 8 * swz(col);        // byte offset of (col, row 0)
-        const int csh = cs + 4 * h;         // the lane's float of it
-        float bv[RM];
-#pragma unroll
-        for (int s = 0; s < RM; ++s)
-          bv[s] = (s < R) ? *reinterpret_cast<const float *>(ldsb + (csh ^ bl.rowslot[s]))
-                          : 0.0f;
-        floatx16 acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f,
-                        0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int s = 0; s < RM; ++s)
-          if (s < R) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(wa[s], bv[s], acc, 0, 0, 0);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int r0 = 4 * q + 2 * h;
-          if (r0 < R)
-            *reinterpret_cast<float2 *>(ldsb + (cs ^ bl.rowslot[r0])) =
-                make_float2(acc[4 * q], acc[4 * q + 1]);
-          if (r0 + 1 < R)
-            *reinterpret_cast<float2 *>(ldsb + (cs ^ bl.rowslot[r0 + 1])) =
-                make_float2(acc[4 * q + 2], acc[4 * q + 3]);
-        }
+      const TileBlock *bl = &blocks[ps.first_block + b];
+      const float *wlb = wl + b * RM * 64;
+      switch (bl->k) {
+        case 1: tp_block<1, NTH, TILE>(ldsb, bl, wlb); break;
+        case 2: tp_block<2, NTH, TILE>(ldsb, bl, wlb); break;
+        case 3: tp_block<3, NTH, TILE>(ldsb, bl, wlb); break;
+        default: tp_block<4, NTH, TILE>(ldsb, bl, wlb); break;
       }
       __syncthreads();
     }
@@ -209,8 +249,9 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
       const int s0 = slot_t ^ swz(it << TBITS);
-      const float2 x = *reinterpret_cast<const float2 *>(ldsb + 8 * s0);
-      const float2 y = *reinterpret_cast<const float2 *>(ldsb + 8 * (s0 ^ 1));
+      const float *re = reinterpret_cast<const float *>(ldsb), *im = re + TILE;
+      const float2 x = make_float2(re[s0], im[s0]);
+      const float2 y = make_float2(re[s0 ^ 1], im[s0 ^ 1]);
       st_nt(a + (base | it_addr(it) | addr_t), make_float4(x.x, x.y, y.x, y.y));
     }
     __syncthreads();
@@ -286,7 +327,7 @@ int tiles_apply(qcs_hip_state *s, const qcs_hip_pass *passes, int np,
         int off = 0;
         for (int q = 0; q < blocks[b].k; ++q)
           if ((r >> q) & 1) off |= 1 << tb[b].tpos[q];
-        tb[b].rowslot[r] = 8 * swz(off);
+        tb[b].rowslot[r] = swz(off);
       }
     }
   }
