@@ -177,7 +177,8 @@ int qcs_hip_apply_unitary(qcs_hip_state *s, int k, const int *targets,
                           const double *u);
 
 /* Tile programs: several fused blocks per HBM pass.  A pass names
- * `tile_bits` index bits (ascending in `bits`, and containing bits 0..5);
+ * `tile_bits` index bits (ascending in `bits`, and containing bits 0..3,
+ * so a tile is read in runs of at least 128 bytes);
  * the state is streamed as 2^tile_bits-amplitude tiles and each of the
  * pass's `num_blocks` consecutive blocks (whose targets lie in `bits`) is
  * applied to every tile in LDS before the tile is written back.  complex64
@@ -186,7 +187,7 @@ int qcs_hip_apply_unitary(qcs_hip_state *s, int k, const int *targets,
  * QCS_HIP_TILE_MAX_BLOCKS blocks (their operands stay in LDS).  No reference
  * counterpart. */
 #define QCS_HIP_TILE_BITS 13
-#define QCS_HIP_TILE_LOW_BITS 6
+#define QCS_HIP_TILE_LOW_BITS 4
 #define QCS_HIP_TILE_KMAX 4
 #define QCS_HIP_TILE_MAX_BLOCKS 23
 typedef struct qcs_hip_pass {

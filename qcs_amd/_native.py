@@ -50,7 +50,7 @@ class HipPass(ctypes.Structure):
                 ("reserved", c_int), ("bits", c_int * 16)]
 
 
-TILE_BITS, TILE_LOW_BITS = 13, 6
+TILE_BITS, TILE_LOW_BITS = 13, 4
 TUNE_TILE_THREADS, TUNE_TILE_GRID = 0, 1
 FUSE_TILES = 0x10
 DEFERRED = 0x20

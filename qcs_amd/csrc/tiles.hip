@@ -1,7 +1,7 @@
 // Tile programs: several fused blocks per HBM pass (complex64, FP32 MFMA).
 //
-// A pass owns a set S of TB = 13 index bits (ascending; always the lowest 6,
-// so a tile is read as >= 512-byte contiguous runs).  The state splits into
+// A pass owns a set S of TB = 13 index bits (ascending; always the lowest 4,
+// so a tile is read as >= 128-byte contiguous runs).  The state splits into
 // 2^(n-13) tiles of 8192 amplitudes; a work-group streams tiles through LDS
 // and, per tile, applies every block of the pass in order -- each block a
 // composed 2^k x 2^k unitary (k <= 4) on k of the tile's bits, applied as a

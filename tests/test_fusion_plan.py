@@ -147,6 +147,9 @@ def test_pass_plan_reproduces_circuit(n, tb, low, k, mb):
 def test_pass_plan_layered_30q_counts():
     passes, blocks, _ = pass_plan(C.layered(30, 20, 2), 30)
     assert (len(passes), len(blocks)) == (38, 189)
+    # the tile programs' default: tiles hold bits 0..3 (128-byte runs)
+    passes, blocks, _ = pass_plan(C.layered(30, 20, 2), 30, low=4)
+    assert (len(passes), len(blocks)) == (26, 196)
 
 
 def test_circuit_gate_counts():
