@@ -87,7 +87,7 @@ def main():
         store[key + "/scr"] = ref.scratch(c)
         L.qc_destroy(c)
 
-    # 4. layered circuit (config-1 shape at small n), then shots
+    # 4. layered circuit (the 20-qubit CPU workload's shape at small n), then shots
     for n, depth, seed in ((6, 3, 0), (8, 2, 5)):
         ops = C.layered(n, depth, seed)
         c = ref.create(n)
