@@ -59,7 +59,9 @@ __global__ __launch_bounds__(NTH) void k_fused_mfma(float2 *__restrict__ a,
   constexpr int IT = (1 << TB) / (2 * NTH);
   constexpr int IB = TB - TBITS;
   static_assert(CT >= 1, "at least one column tile per wave");
-  __shared__ float2 lds[R * CS];
+  // real and imaginary parts in separate planes: a half-wave reads one part
+  // of 32 consecutive columns, 32 distinct 4-byte banks
+  __shared__ float lre[R * CS], lim[R * CS];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int h = lane >> 5, l32 = lane & 31;
@@ -124,7 +126,7 @@ __global__ __launch_bounds__(NTH) void k_fused_mfma(float2 *__restrict__ a,
 #pragma unroll
     for (int it = 0; it < IT; ++it) v[it] = bp_ld(a + (base | addr_it[it]));
   }
-  const float *ldsf = reinterpret_cast<const float *>(lds);
+  const float *bpl = h ? lim : lre;  // this lane's part of the B operand
   for (; tile < end; tile += step) {
     if (!PIPE) {
 #pragma unroll
@@ -132,8 +134,10 @@ __global__ __launch_bounds__(NTH) void k_fused_mfma(float2 *__restrict__ a,
     }
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
-      lds[pos_it[it]] = make_float2(v[it].x, v[it].y);
-      lds[pos_it[it] + w0] = make_float2(v[it].z, v[it].w);
+      lre[pos_it[it]] = v[it].x;
+      lim[pos_it[it]] = v[it].y;
+      lre[pos_it[it] + w0] = v[it].z;
+      lim[pos_it[it] + w0] = v[it].w;
     }
     __syncthreads();
     const uint64_t next = tile + step;
@@ -147,7 +151,7 @@ __global__ __launch_bounds__(NTH) void k_fused_mfma(float2 *__restrict__ a,
       const int col = (wave * CT + ct) * 32 + l32;
       float b[R];
 #pragma unroll
-      for (int s = 0; s < R; ++s) b[s] = ldsf[(s * CS + col) * 2 + h];
+      for (int s = 0; s < R; ++s) b[s] = bpl[s * CS + col];
       floatx16 acc[MT];
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt) {
@@ -162,16 +166,22 @@ __global__ __launch_bounds__(NTH) void k_fused_mfma(float2 *__restrict__ a,
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int r0 = mt * 16 + 4 * q + 2 * h;
-          if (r0 < R) lds[r0 * CS + col] = make_float2(acc[mt][4 * q], acc[mt][4 * q + 1]);
-          if (r0 + 1 < R)
-            lds[(r0 + 1) * CS + col] = make_float2(acc[mt][4 * q + 2], acc[mt][4 * q + 3]);
+          if (r0 < R) {
+            lre[r0 * CS + col] = acc[mt][4 * q];
+            lim[r0 * CS + col] = acc[mt][4 * q + 1];
+          }
+          if (r0 + 1 < R) {
+            lre[(r0 + 1) * CS + col] = acc[mt][4 * q + 2];
+            lim[(r0 + 1) * CS + col] = acc[mt][4 * q + 3];
+          }
         }
     }
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
-      const float2 x = lds[pos_it[it]], y = lds[pos_it[it] + w0];
-      bp_st(a + (base | addr_it[it]), make_float4(x.x, x.y, y.x, y.y));
+      const int p = pos_it[it];
+      bp_st(a + (base | addr_it[it]),
+            make_float4(lre[p], lim[p], lre[p + w0], lim[p + w0]));
     }
     __syncthreads();
     base = nbase;
