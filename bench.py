@@ -202,7 +202,9 @@ def bench_batch(args):
     splitmix64 seed 3; even circuits complex128 plain, odd fused complex64),
     256 shots each, one qc_batch_sample call per step (host planning, one
     persistent device launch replayed from a hipGraph after the first step,
-    sampled indices back on the host)."""
+    sampled indices back on the host).  Timed steps plan from scratch
+    (QCS_BATCH_REPLAN); the reuse of an unchanged batch's plan (new uniforms
+    only) is timed beside them as rerun_ms_per_step."""
     import qcs_amd
     from qcs_amd import circuits as C
 
@@ -239,7 +241,7 @@ def bench_batch(args):
     b0.close()
     print(f"batch: first run {first_s:.2f} s, device {first_dev_ms:.1f} ms, "
           f"kernel {kernel_ms:.1f} ms", file=sys.stderr, flush=True)
-    b = qcs_amd.Batch(qcs_amd.BATCH_GRAPH)
+    b = qcs_amd.Batch(qcs_amd.BATCH_GRAPH | qcs_amd.BATCH_REPLAN)
     b.set_threads(args.batch_threads)
     if not args.no_tune:
         t0 = time.perf_counter()
@@ -256,6 +258,23 @@ def bench_batch(args):
         totals.append(b.last_timing()[1])
     hip.qcs_hip_device_sync()
     T = (time.perf_counter() - t1) / args.steps
+    b.close()
+    # the same circuits sampled again with the plan of the previous run
+    br = qcs_amd.Batch(qcs_amd.BATCH_GRAPH)
+    br.set_threads(args.batch_threads)
+    if not args.no_tune:
+        br.autotune(circs)
+    for _ in range(2):
+        br.sample(circs, shots)
+    hip.qcs_hip_device_sync()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        br.sample(circs, shots)
+    hip.qcs_hip_device_sync()
+    T_rerun = (time.perf_counter() - t1) / args.steps
+    br.close()
+    print(f"batch: {T * 1e3:.1f} ms per planned step, {T_rerun * 1e3:.1f} ms per rerun",
+          file=sys.stderr, flush=True)
     total_amp = amp_updates[False] + amp_updates[True]
     dev_ms = sum(totals) / len(totals)
     result = {
@@ -282,6 +301,7 @@ def bench_batch(args):
         },
         "device_ms_per_step": dev_ms,
         "host_ms_per_step": T * 1e3 - dev_ms,
+        "rerun_ms_per_step": T_rerun * 1e3,
         "kernel_ms": kernel_ms,
         "device_amp_updates_per_s_kernel": total_amp / (kernel_ms * 1e-3),
         "device_amp_updates_per_s": total_amp / (dev_ms * 1e-3),
@@ -292,7 +312,6 @@ def bench_batch(args):
     }
     for c in circs:
         c.close()
-    b.close()
     print(json.dumps(result), flush=True)
 
 

@@ -126,7 +126,8 @@ void qc_grover_search_iters(t_q_circuit *circuit, long solution_state,
 void qc_sample(t_q_circuit *circuit, int shots, long *indices);
 
 /* Reset the state to the basis state |index> (recorded gates are dropped;
- * the history is kept). */
+ * the history is kept).  On a QCS_DEFERRED circuit (index 0 only) it drops
+ * the recorded gates, so the circuit can be recorded again. */
 void qc_set_basis_state(t_q_circuit *circuit, long index);
 
 /* Amplitude access: 2*count doubles {re, im} from index `offset`. */
@@ -159,6 +160,12 @@ typedef struct s_q_batch t_q_batch;
                                      (counts and sizes) is unchanged */
 #define QCS_BATCH_KEEP_STATES 0x2 /* keep the final states for
                                      qc_batch_get_amplitudes */
+#define QCS_BATCH_REPLAN 0x4      /* plan every run from scratch; by default
+                                     a run of the same circuits (unchanged
+                                     since, same shot count) as the last run
+                                     reuses that run's plan and device
+                                     descriptors and only draws new
+                                     uniforms */
 t_q_batch *qc_batch_create(int options);
 void qc_batch_destroy(t_q_batch *batch);
 /* indices[i * shots + s]: shot s of circuits[i] (-1: dropped).  Uniforms

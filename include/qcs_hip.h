@@ -295,6 +295,14 @@ int qcs_hip_batch_run(qcs_hip_batch *b, const qcs_hip_batch_circuit *circuits,
                       int num_passes, const qcs_hip_block *blocks,
                       int num_blocks, const qcs_hip_gate *gates, int num_gates,
                       long shots, const double *u, long *index_out);
+/* Repeat the last successful qcs_hip_batch_run with new uniforms: the
+ * descriptors and composed blocks already on the device are reused, only
+ * num_circuits * shots uniforms travel.  shots must equal the last run's;
+ * fails with QCS_HIP_EINVAL when there is no such run (none yet, the last
+ * one failed, or the work-group size changed since).  No reference
+ * counterpart: for callers that sample unchanged circuits again. */
+int qcs_hip_batch_rerun(qcs_hip_batch *b, long shots, const double *u,
+                        long *index_out);
 /* amplitudes [offset, offset + count) of circuit i's final state after the
  * last run (QCS_HIP_BATCH_KEEP_STATES only) as {re, im} doubles */
 int qcs_hip_batch_read_state(qcs_hip_batch *b, int circuit, long offset,

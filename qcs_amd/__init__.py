@@ -10,7 +10,7 @@ import ctypes
 import numpy as np
 
 from . import circuits
-from ._native import (BATCH_GRAPH, BATCH_KEEP_STATES, C64, C128, DEFERRED,
+from ._native import (BATCH_GRAPH, BATCH_KEEP_STATES, BATCH_REPLAN, C64, C128, DEFERRED,
                       FUSE_TILES, FUSED_FP32, FUSE_K, HIP_COMPAT, K_1Q,
                       K_COMPOSE, K_COUNT, K_CU, K_FUSED, K_OTHER, K_REDUCE,
                       PLAIN_FP32, PLAIN_FP64, REFERENCE_COMPAT,
@@ -18,7 +18,8 @@ from ._native import (BATCH_GRAPH, BATCH_KEEP_STATES, C64, C128, DEFERRED,
                       HipBatchCircuit, HipBlock, HipGate, HipPass, libc, load)
 
 __all__ = ["Circuit", "Batch", "load", "libc", "circuits", "PLAIN_FP64", "PLAIN_FP32",
-           "DEFERRED", "BATCH_GRAPH", "BATCH_KEEP_STATES", "HipBatchCircuit",
+           "DEFERRED", "BATCH_GRAPH", "BATCH_KEEP_STATES", "BATCH_REPLAN",
+           "HipBatchCircuit",
            "FUSED_FP32", "REFERENCE_COMPAT", "RENORM_AT_BARRIER", "FUSE_K",
            "FUSE_TILES", "HipPass", "TUNE_TILE_THREADS", "TUNE_TILE_GRID",
            "C128", "C64", "HIP_COMPAT", "HipGate", "HipBlock", "K_1Q", "K_CU",

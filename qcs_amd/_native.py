@@ -60,7 +60,7 @@ BATCH_MAX_QUBITS = 22
 BATCH_C128_TILE_BITS, BATCH_C64_TILE_BITS = 13, 14
 HIP_BATCH_KEEP_STATES, HIP_BATCH_GRAPH = 1, 2
 TUNE_BATCH_THREADS = 0
-BATCH_GRAPH, BATCH_KEEP_STATES = 0x1, 0x2
+BATCH_GRAPH, BATCH_KEEP_STATES, BATCH_REPLAN = 0x1, 0x2, 0x4
 
 
 class HipBatchCircuit(ctypes.Structure):
@@ -111,6 +111,7 @@ _HIP_PROTOS = {
     "qcs_hip_batch_run": (c_int, [c_void_p, P(HipBatchCircuit), c_int, P(HipPass), c_int,
                                   P(HipBlock), c_int, P(HipGate), c_int, c_long,
                                   P(c_double), P(c_long)]),
+    "qcs_hip_batch_rerun": (c_int, [c_void_p, c_long, P(c_double), P(c_long)]),
     "qcs_hip_batch_read_state": (c_int, [c_void_p, c_int, c_long, c_long, P(c_double)]),
     "qcs_hip_batch_last_timing": (c_int, [c_void_p, P(c_double), P(c_double), P(c_int)]),
 }

@@ -809,9 +809,15 @@ __global__ void k_batch_reset(unsigned *queue) {
   if (threadIdx.x == 0) queue[0] = 0u;
 }
 
-}  // namespace qcs
-
 // ---- host side ---------------------------------------------------------------------
+
+// byte offsets of the regions of the descriptor block (device and pinned
+// staging alike)
+struct BatchLayout {
+  size_t circ, order, pass, gate, group, block, hgate, hblock, u, total;
+};
+
+}  // namespace qcs
 
 struct qcs_hip_batch {
   int flags = 0;
@@ -840,6 +846,13 @@ struct qcs_hip_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<long> graph_key;
+  hipGraph_t rerun_graph = nullptr;  // the same run with new uniforms only
+  hipGraphExec_t rerun_exec = nullptr;
+  // launch of the last successful run, repeated by qcs_hip_batch_rerun
+  bool has_last = false;
+  qcs::BatchLayout last_layout = {};
+  int last_nc = 0, last_tile_bytes = 0, last_nch_cap = 0, last_grid = 0;
+  long last_shots = 0;
   // last run, for read_state and timing
   std::vector<int> last_n, last_prec;
   std::vector<uint64_t> last_off;
@@ -849,17 +862,17 @@ struct qcs_hip_batch {
 
 namespace qcs {
 
-struct BatchLayout {
-  size_t circ, order, pass, gate, group, block, hgate, hblock, u, total;
-};
-
 static size_t align_up(size_t x) { return (x + kBAlign - 1) / kBAlign * kBAlign; }
 
 static void drop_graph(qcs_hip_batch *b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->rerun_exec) (void)hipGraphExecDestroy(b->rerun_exec);
+  if (b->rerun_graph) (void)hipGraphDestroy(b->rerun_graph);
   b->exec = nullptr;
   b->graph = nullptr;
+  b->rerun_exec = nullptr;
+  b->rerun_graph = nullptr;
   b->graph_key.clear();
 }
 
@@ -1106,15 +1119,21 @@ static int batch_launch(qcs_hip_batch *b, int nth, const BatchLayout &L, int nc,
 }
 
 // The launch sequence of one run on b->stream: uploads, composition, queue
-// reset, the persistent kernel, the download of the sampled indices.
+// reset, the persistent kernel, the download of the sampled indices.  A
+// rerun uploads only the uniforms and keeps the composed blocks.
 static int enqueue_run(qcs_hip_batch *b, const BatchLayout &L, int nc, int nb,
                        uint64_t shots, int tile_bytes, int nch_cap, int grid,
-                       bool timed) {
+                       bool timed, bool rerun) {
   char *d = static_cast<char *>(b->d_desc);
-  QCS_HIP_TRY(hipMemcpyAsync(d, b->h_stage, L.total, hipMemcpyHostToDevice,
-                             b->stream));
+  const size_t ubytes = sizeof(double) * size_t(nc) * size_t(shots);
+  if (!rerun)
+    QCS_HIP_TRY(hipMemcpyAsync(d, b->h_stage, L.total, hipMemcpyHostToDevice,
+                               b->stream));
+  else if (ubytes)
+    QCS_HIP_TRY(hipMemcpyAsync(d + L.u, b->h_stage + L.u, ubytes,
+                               hipMemcpyHostToDevice, b->stream));
   int rc;
-  if (nb > 0 &&
+  if (!rerun && nb > 0 &&
       (rc = launch_compose(b->stream,
                            reinterpret_cast<const qcs_hip_gate *>(d + L.hgate),
                            reinterpret_cast<const qcs_hip_block *>(d + L.hblock),
@@ -1126,10 +1145,62 @@ static int enqueue_run(qcs_hip_batch *b, const BatchLayout &L, int nc, int nb,
   if ((rc = batch_launch(b, b->nth, L, nc, shots, tile_bytes, nch_cap, grid)))
     return rc;
   if (timed) QCS_HIP_TRY(hipEventRecord(b->ev[2], b->stream));
-  const size_t obytes = sizeof(long) * size_t(nc) * size_t(shots);
-  if (obytes)
-    QCS_HIP_TRY(hipMemcpyAsync(b->h_out, b->d_out, obytes, hipMemcpyDeviceToHost,
+  if (ubytes)
+    QCS_HIP_TRY(hipMemcpyAsync(b->h_out, b->d_out, ubytes, hipMemcpyDeviceToHost,
                                b->stream));
+  return 0;
+}
+
+// Captures the launch sequence as a graph (graph mode: on first use, then
+// replays it) or enqueues it directly with the kernel timed, waits, and
+// records the device times.
+static int submit(qcs_hip_batch *b, bool rerun, int nb, bool build_graph) {
+  const BatchLayout &L = b->last_layout;
+  const int nc = b->last_nc;
+  const uint64_t shots = uint64_t(b->last_shots);
+  const bool use_graph = (b->flags & QCS_HIP_BATCH_GRAPH) != 0;
+  hipGraph_t *graph = rerun ? &b->rerun_graph : &b->graph;
+  hipGraphExec_t *exec = rerun ? &b->rerun_exec : &b->exec;
+  int rc;
+  QCS_HIP_TRY(hipEventRecord(b->ev[0], b->stream));
+  b->replayed = 0;
+  if (use_graph) {
+    if (!*exec || build_graph) {
+      if (*exec) (void)hipGraphExecDestroy(*exec);
+      if (*graph) (void)hipGraphDestroy(*graph);
+      *exec = nullptr;
+      *graph = nullptr;
+      QCS_HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      rc = enqueue_run(b, L, nc, nb, shots, b->last_tile_bytes, b->last_nch_cap,
+                       b->last_grid, false, rerun);
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(b->stream, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (e != hipSuccess) return check_hip(e, "hipStreamEndCapture");
+      *graph = g;
+      QCS_HIP_TRY(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+    } else {
+      b->replayed = 1;
+    }
+    QCS_HIP_TRY(hipGraphLaunch(*exec, b->stream));
+  } else {
+    if ((rc = enqueue_run(b, L, nc, nb, shots, b->last_tile_bytes, b->last_nch_cap,
+                          b->last_grid, true, rerun)))
+      return rc;
+  }
+  QCS_HIP_TRY(hipEventRecord(b->ev[3], b->stream));
+  QCS_HIP_TRY(hipStreamSynchronize(b->stream));
+  float ms = 0.0f;
+  QCS_HIP_TRY(hipEventElapsedTime(&ms, b->ev[0], b->ev[3]));
+  b->total_ms = ms;
+  b->kernel_ms = -1.0f;
+  if (!use_graph) {
+    QCS_HIP_TRY(hipEventElapsedTime(&ms, b->ev[1], b->ev[2]));
+    b->kernel_ms = ms;
+  }
   return 0;
 }
 
@@ -1156,6 +1227,7 @@ int batch_run(qcs_hip_batch *b, const qcs_hip_batch_circuit *cs, int nc,
   int rc = validate_batch(cs, nc, passes, np, blocks, nb, gates, ng, shots, u,
                           out);
   if (rc) return rc;
+  b->has_last = false;
   if (nc == 0) return 0;
   // a pass may serve several circuits, all of one precision and tile size;
   // the first circuit that names it owns it
@@ -1376,47 +1448,20 @@ int batch_run(qcs_hip_batch *b, const qcs_hip_batch_circuit *cs, int nc,
   if (per_cu < 1) return fail(QCS_HIP_EDEVICE, "batch kernel does not fit on a CU");
   const int grid = std::min(nc, per_cu * b->num_cus);
 
-  const bool use_graph = (b->flags & QCS_HIP_BATCH_GRAPH) != 0;
   std::vector<long> key = {b->generation, nc, np, nb, long(cgate.size()), long(hg.size()),
                            long(hgrp.size()), long(hb.size()), shots, b->nth,
-                           lds_bytes, grid, long(L.total), long(states_bytes)};
-  QCS_HIP_TRY(hipEventRecord(b->ev[0], b->stream));
-  b->replayed = 0;
-  if (use_graph) {
-    if (!b->exec || b->graph_key != key) {
-      drop_graph(b);
-      QCS_HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-      rc = enqueue_run(b, L, nc, nb, uint64_t(shots), max_tile_bytes, nch_cap,
-                       grid, false);
-      hipGraph_t g = nullptr;
-      const hipError_t e = hipStreamEndCapture(b->stream, &g);
-      if (rc) {
-        if (g) (void)hipGraphDestroy(g);
-        return rc;
-      }
-      if (e != hipSuccess) return check_hip(e, "hipStreamEndCapture");
-      b->graph = g;
-      QCS_HIP_TRY(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
-      b->graph_key = key;
-    } else {
-      b->replayed = 1;
-    }
-    QCS_HIP_TRY(hipGraphLaunch(b->exec, b->stream));
-  } else {
-    if ((rc = enqueue_run(b, L, nc, nb, uint64_t(shots), max_tile_bytes, nch_cap,
-                          grid, true)))
-      return rc;
-  }
-  QCS_HIP_TRY(hipEventRecord(b->ev[3], b->stream));
-  QCS_HIP_TRY(hipStreamSynchronize(b->stream));
-  float ms = 0.0f;
-  QCS_HIP_TRY(hipEventElapsedTime(&ms, b->ev[0], b->ev[3]));
-  b->total_ms = ms;
-  b->kernel_ms = -1.0f;
-  if (!use_graph) {
-    QCS_HIP_TRY(hipEventElapsedTime(&ms, b->ev[1], b->ev[2]));
-    b->kernel_ms = ms;
-  }
+                           max_tile_bytes, nch_cap, grid, long(L.total),
+                           long(states_bytes)};
+  const bool new_graph = (b->flags & QCS_HIP_BATCH_GRAPH) && b->graph_key != key;
+  if (new_graph) drop_graph(b);
+  b->last_layout = L;
+  b->last_nc = nc;
+  b->last_shots = shots;
+  b->last_tile_bytes = max_tile_bytes;
+  b->last_nch_cap = nch_cap;
+  b->last_grid = grid;
+  if ((rc = submit(b, false, nb, new_graph))) return rc;
+  if (b->flags & QCS_HIP_BATCH_GRAPH) b->graph_key = key;
   const double t3 = host_ms();
   if (shots > 0)
     std::memcpy(out, b->h_out, sizeof(long) * size_t(nc) * size_t(shots));
@@ -1433,6 +1478,25 @@ int batch_run(qcs_hip_batch *b, const qcs_hip_batch_circuit *cs, int nc,
     b->last_n[size_t(i)] = cs[i].num_qubits;
     b->last_prec[size_t(i)] = cs[i].precision;
   }
+  b->has_last = true;
+  return 0;
+}
+
+int batch_rerun(qcs_hip_batch *b, long shots, const double *u, long *out) {
+  if (!b->has_last) return fail(QCS_HIP_EINVAL, "no successful run to repeat");
+  if (shots != b->last_shots)
+    return fail(QCS_HIP_EINVAL, "shots %ld differ from the last run's %ld", shots,
+                b->last_shots);
+  const size_t n = size_t(b->last_nc) * size_t(shots);
+  if (n > 0 && (!u || !out)) return fail(QCS_HIP_EINVAL, "null uniforms or output");
+  QCS_HIP_TRY(hipSetDevice(b->device));
+  if (n > 0) std::memcpy(b->h_stage + b->last_layout.u, u, sizeof(double) * n);
+  int rc = submit(b, true, 0, false);
+  if (rc) {
+    b->has_last = false;
+    return rc;
+  }
+  if (n > 0) std::memcpy(out, b->h_out, sizeof(long) * n);
   return 0;
 }
 
@@ -1495,7 +1559,10 @@ int qcs_hip_batch_set_tuning(qcs_hip_batch *b, int knob, int value) {
   if (knob == QCS_HIP_TUNE_BATCH_THREADS) {
     if (value != 256 && value != 512)
       return fail(QCS_HIP_EINVAL, "batch threads %d: use 256 or 512", value);
-    if (value != b->nth) drop_graph(b);
+    if (value != b->nth) {
+      drop_graph(b);
+      b->has_last = false;  // the grid was sized for the old work-group
+    }
     b->nth = value;
     return 0;
   }
@@ -1510,6 +1577,12 @@ int qcs_hip_batch_run(qcs_hip_batch *b, const qcs_hip_batch_circuit *circuits,
   if (!b) return fail(QCS_HIP_EINVAL, "null batch");
   return batch_run(b, circuits, num_circuits, passes, num_passes, blocks,
                    num_blocks, gates, num_gates, shots, u, index_out);
+}
+
+int qcs_hip_batch_rerun(qcs_hip_batch *b, long shots, const double *u,
+                        long *index_out) {
+  if (!b) return fail(QCS_HIP_EINVAL, "null batch");
+  return batch_rerun(b, shots, u, index_out);
 }
 
 int qcs_hip_batch_read_state(qcs_hip_batch *b, int circuit, long offset,

@@ -12,6 +12,9 @@
  *     or qcs_pass_plan when the state is larger than a tile).
  * Uniforms come from rand() circuit after circuit, shot after shot, which is
  * the stream qc_sample on each circuit in turn would draw (src/qcs.c:596).
+ * A run of the same circuits (same pointers and recorded-gate versions) with
+ * the same shot count as the batch's last run reuses that run's device
+ * descriptors (qcs_hip_batch_rerun): only the new uniforms travel.
  */
 #define _POSIX_C_SOURCE 200112L
 #include <limits.h>
@@ -35,12 +38,16 @@ struct s_q_batch {
   /* per qubit count: tile bits [plain, fused] and fusion width; 0 = default */
   signed char tile_bits[2][QCS_HIP_BATCH_MAX_QUBITS + 1];
   signed char fuse_k[QCS_HIP_BATCH_MAX_QUBITS + 1];
+  /* the last successful run: its circuits and their versions */
+  t_q_circuit **last_circ;
+  long *last_version;
+  int last_count, last_cap, last_shots, last_valid;
 };
 
 t_q_batch *qc_batch_create(int options) {
   t_q_batch *b;
   int flags = 0;
-  if (options & ~(QCS_BATCH_GRAPH | QCS_BATCH_KEEP_STATES)) {
+  if (options & ~(QCS_BATCH_GRAPH | QCS_BATCH_KEEP_STATES | QCS_BATCH_REPLAN)) {
     fprintf(stderr, "Error: unknown batch options %d.\n", options);
     return NULL;
   }
@@ -60,6 +67,8 @@ t_q_batch *qc_batch_create(int options) {
 void qc_batch_destroy(t_q_batch *batch) {
   if (!batch) return;
   qcs_hip_batch_destroy(batch->dev);
+  free(batch->last_circ);
+  free(batch->last_version);
   free(batch);
 }
 
@@ -72,12 +81,53 @@ int qc_batch_set_plan(t_q_batch *batch, int fused, int num_qubits,
     return QCS_HIP_EINVAL;
   batch->tile_bits[fused ? 1 : 0][num_qubits] = (signed char)tile_bits;
   if (fused) batch->fuse_k[num_qubits] = (signed char)fuse_k;
+  batch->last_valid = 0;
   return 0;
 }
 
 int qc_batch_set_threads(t_q_batch *batch, int threads) {
   if (!batch) return QCS_HIP_EINVAL;
+  batch->last_valid = 0;
   return qcs_hip_batch_set_tuning(batch->dev, QCS_HIP_TUNE_BATCH_THREADS, threads);
+}
+
+/* 1 when (circuits, shots) is the batch's last successful run unchanged */
+static int same_as_last(const t_q_batch *b, t_q_circuit **circuits, int count,
+                        int shots) {
+  int i;
+  if (!b->last_valid || (b->options & QCS_BATCH_REPLAN) ||
+      count != b->last_count || shots != b->last_shots)
+    return 0;
+  for (i = 0; i < count; i++)
+    if (circuits[i] != b->last_circ[i] ||
+        circuits[i]->version != b->last_version[i])
+      return 0;
+  return 1;
+}
+
+static void remember(t_q_batch *b, t_q_circuit **circuits, int count,
+                     int shots) {
+  int i;
+  b->last_valid = 0;
+  if (b->options & QCS_BATCH_REPLAN) return;
+  if (count > b->last_cap) {
+    t_q_circuit **lc = (t_q_circuit **)realloc(b->last_circ,
+                                               (size_t)count * sizeof(*lc));
+    long *lv;
+    if (!lc) return;
+    b->last_circ = lc;
+    lv = (long *)realloc(b->last_version, (size_t)count * sizeof(*lv));
+    if (!lv) return;
+    b->last_version = lv;
+    b->last_cap = count;
+  }
+  for (i = 0; i < count; i++) {
+    b->last_circ[i] = circuits[i];
+    b->last_version[i] = circuits[i]->version;
+  }
+  b->last_count = count;
+  b->last_shots = shots;
+  b->last_valid = 1;
 }
 
 static int plan_tile_bits(const t_q_batch *b, int fused, int n) {
@@ -251,9 +301,28 @@ int qc_batch_sample(t_q_batch *batch, t_q_circuit **circuits, int count,
       return QCS_HIP_EINVAL;
     }
   if (count == 0) return 0;
+  total = (long)count * (long)shots;
+  if (same_as_last(batch, circuits, count, shots)) {
+    if (total > 0) {
+      u = (double *)malloc((size_t)total * sizeof(double));
+      if (!u) return QCS_HIP_ENOMEM;
+    }
+    for (s = 0; s < total; s++) u[s] = rand() / (double)RAND_MAX;
+    t0 = now_ms();
+    rc = qcs_hip_batch_rerun(batch->dev, shots, u, indices);
+    if (profiling())
+      fprintf(stderr, "qc_batch_sample: rerun of the last plan %.1f ms\n",
+              now_ms() - t0);
+    if (rc != 0) {
+      batch->last_valid = 0;
+      fprintf(stderr, "Error: batch run failed: %s\n", qcs_hip_last_error());
+    }
+    free(u);
+    return rc;
+  }
+  batch->last_valid = 0;
   plans = (circuit_plan *)calloc((size_t)count, sizeof(circuit_plan));
   circ = (qcs_hip_batch_circuit *)calloc((size_t)count, sizeof(qcs_hip_batch_circuit));
-  total = (long)count * (long)shots;
   if (total > 0) u = (double *)malloc((size_t)total * sizeof(double));
   if (!plans || !circ || (total > 0 && !u)) goto out;
   t0 = now_ms();
@@ -314,6 +383,8 @@ int qc_batch_sample(t_q_batch *batch, t_q_circuit **circuits, int count,
             t2 - t1, t3 - t2, now_ms() - t3);
   if (rc != 0)
     fprintf(stderr, "Error: batch run failed: %s\n", qcs_hip_last_error());
+  else
+    remember(batch, circuits, count, shots);
 out:
   if (plans)
     for (i = 0; i < count; i++) free_plan(&plans[i]);

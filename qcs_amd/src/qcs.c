@@ -123,9 +123,16 @@ static void add_history(t_q_circuit *c, const char *name, int target,
 /* recording and flushing (fused form)                                     */
 /* ---------------------------------------------------------------------- */
 
+static long g_version_stamp = 0;
+
+void qcs_touch(t_q_circuit *c) {
+  if (c) c->version = ++g_version_stamp;
+}
+
 static void record(t_q_circuit *c, int kind, int target, int control,
                    const double m[8]) {
   qcs_hip_gate *g;
+  qcs_touch(c);
   if (c->pending_size >= c->pending_capacity) {
     int cap = c->pending_capacity ? c->pending_capacity * 2 : 256;
     qcs_hip_gate *p =
@@ -199,6 +206,7 @@ static void flush_pending(t_q_circuit *c) {
   free(blocks);
   free(ordered);
   c->pending_size = 0;
+  qcs_touch(c);
 }
 
 /* ---------------------------------------------------------------------- */
@@ -246,6 +254,7 @@ t_q_circuit *qc_create_ex(int num_qubits, int options) {
   c = (t_q_circuit *)malloc(sizeof(t_q_circuit));
   if (!c) return NULL;
   memset(c, 0, sizeof(*c));
+  qcs_touch(c);
   c->num_qubits = num_qubits;
   c->options = options;
   c->history_capacity = 100;
@@ -523,7 +532,19 @@ double qc_get_probability(t_q_circuit *circuit, int state) {
 }
 
 void qc_set_basis_state(t_q_circuit *circuit, long index) {
-  if (!circuit || !circuit->state) return;
+  if (!circuit) return;
+  if (is_deferred(circuit)) {
+    /* a deferred circuit always starts from |0...0>: drop its recorded
+       gates so it can be recorded again (e.g. with new angles) */
+    if (index != 0) {
+      fprintf(stderr, "Error: a deferred circuit starts from |0...0>\n");
+      return;
+    }
+    circuit->pending_size = 0;
+    qcs_touch(circuit);
+    return;
+  }
+  if (!circuit->state) return;
   if (index < 0 || index >= (1L << circuit->num_qubits)) {
     fprintf(stderr, "Error: Invalid state or basis index\n");
     return;
@@ -867,5 +888,6 @@ void qc_optimize(t_q_circuit *circuit) {
     }
     circuit->pending[top++] = *g;
   }
+  if (top != circuit->pending_size) qcs_touch(circuit);
   circuit->pending_size = top;
 }

@@ -25,7 +25,12 @@ struct t_q_circuit {
   long fused_blocks;
   long fused_passes;
   int tuned_k; /* fusion width chosen by qc_autotune (0: none) */
+  long version; /* changes whenever the recorded gates change (a stamp
+                   unique across circuits, for batch plan reuse) */
 };
+
+/* mark a circuit's recorded gates as changed (qcs.c) */
+void qcs_touch(t_q_circuit *c);
 
 /* Greedy fusion planner (fusion.c).
  * Input: `ng` gates in circuit order on `n` qubits.  Output: `*nblocks`

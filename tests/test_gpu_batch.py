@@ -215,3 +215,50 @@ def test_optimize_cancels_recorded_pairs(qlib, cl):
         e.apply(ops)
         omf, _ = f.compare_with(e)
         assert omf <= 1e-6
+
+
+@pytest.mark.parametrize("graph", [0, qcs_amd.BATCH_GRAPH])
+def test_rerun_reuses_the_last_plan(qlib, cl, capfd, monkeypatch, graph):
+    # sampling the same, unchanged circuits again reuses the device plan and
+    # only draws new uniforms; every run must equal a batch that plans from
+    # scratch each time (QCS_BATCH_REPLAN) on the same rand() stream
+    monkeypatch.setenv("QCS_BATCH_PROFILE", "1")
+    circs = []
+    for n in (2, 9, 13, 15, 17):
+        for opt in (qcs_amd.PLAIN_FP64, qcs_amd.FUSED_FP32):
+            c = qcs_amd.Circuit(n, qcs_amd.DEFERRED | opt)
+            c.apply(make_ops(n, 500 + n))
+            circs.append(c)
+    opts = qcs_amd.BATCH_KEEP_STATES | graph
+    b = qcs_amd.Batch(opts)
+    r = qcs_amd.Batch(opts | qcs_amd.BATCH_REPLAN)
+
+    def both(seed, shots, reruns):
+        cl.srand(seed)
+        got = [b.sample(circs, shots) for _ in range(3)]
+        assert capfd.readouterr().err.count("rerun of the last plan") == reruns
+        cl.srand(seed)
+        want = [r.sample(circs, shots) for _ in range(3)]
+        assert "rerun" not in capfd.readouterr().err
+        for g, w in zip(got, want):
+            assert np.array_equal(g, w)
+        for i, c in enumerate(circs):
+            assert np.array_equal(b.amplitudes(i, c.n), r.amplitudes(i, c.n))
+        return got
+
+    got = both(21, 40, 2)
+    assert not np.array_equal(got[0], got[1])  # new uniforms each run
+    # re-recorded circuit: planned again, then reused
+    circs[3].set_basis_state(0)
+    circs[3].apply(make_ops(9, 77))
+    both(5, 40, 2)
+    # another shot count: planned again (the first of the three)
+    both(6, 24, 2)
+    # a changed work-group size invalidates the plan
+    b.set_threads(256)
+    r.set_threads(256)
+    both(7, 24, 2)
+    for c in circs:
+        c.close()
+    b.close()
+    r.close()
