@@ -37,6 +37,7 @@ sys.path.insert(0, ROOT)
 METRIC = "amplitude updates/s at 30 qubits: fused FP32 vs plain FP64, at 1-F<=1e-6"
 HBM_PEAK_GBS = 8000.0
 FP32_MFMA_PEAK_TFLOPS = 157.3
+BF16_MFMA_PEAK_TFLOPS = 2500.0
 
 
 def parse():
@@ -49,8 +50,9 @@ def parse():
     p.add_argument("--seed", type=int, default=2)
     p.add_argument("--fuse-k", type=int, default=0,
                    help="max qubits per fused block (default: 4 in tile programs, 5 per-block)")
-    p.add_argument("--mode", choices=["tiles", "blocks"], default="blocks",
-                   help="tiles: several fused blocks per HBM pass; blocks: one pass per block")
+    p.add_argument("--mode", choices=["tiles", "split", "blocks"], default="blocks",
+                   help="tiles: several fused blocks per HBM pass; split: tiles whose "
+                        "blocks run as split-bf16 products; blocks: one pass per block")
     p.add_argument("--tile-threads", type=int, default=1024)
     p.add_argument("--no-plain", action="store_true", help="skip the plain FP64 run")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -342,9 +344,12 @@ def main():
         opts |= qcs_amd.FUSE_K(args.fuse_k)
     if args.mode == "tiles":
         opts |= qcs_amd.FUSE_TILES
+    if args.mode == "split":
+        opts |= qcs_amd.FUSE_SPLIT_BF16
+    tiled = args.mode in ("tiles", "split")
     fused = qcs_amd.Circuit(n, opts)
     hip.qcs_hip_set_tuning(fused.device_state, qcs_amd.TUNE_TILE_THREADS, args.tile_threads)
-    fuse_k = args.fuse_k or (4 if args.mode == "tiles" else 5)
+    fuse_k = args.fuse_k or (4 if tiled else 5)
     if not args.no_tune and not args.fuse_k:
         # kernel variant per (n, k) and fusion width for this circuit
         fused.apply(ops)
@@ -390,14 +395,17 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": ("f32 state, split-bf16 MFMA products, f32 accumulation"
+                  if args.mode == "split" else "f32"),
         "data": "synthetic",
         "config": {
             "workload": (f"{n}-qubit layered Rz-Ry-Rz + CNOT-ladder circuit, depth "
                          f"{depth}, angles splitmix64(seed {args.seed}), complex64 "
                          f"state ({(8 << n) / 2**30:.0f} GiB), gates fused into "
-                         f"k<={fuse_k} unitaries on FP32 MFMA, "
-                         + ("several per HBM pass (tile programs)" if args.mode == "tiles"
+                         f"k<={fuse_k} unitaries on "
+                         + ("bf16 MFMA as split products (matrix 3 parts, state 2), "
+                            if args.mode == "split" else "FP32 MFMA, ")
+                         + ("several per HBM pass (tile programs)" if tiled
                             else "one HBM pass each")),
             "qubits": n,
             "depth": depth,
@@ -416,7 +424,7 @@ def main():
     result["roofline"] = {
         "bound": "hbm",
         "kernel": ("k_tile_program (complex64, fused blocks per tile pass)"
-                   if args.mode == "tiles" else "k_fused_mfma (complex64 fused k-qubit pass)"),
+                   if tiled else "k_fused_mfma (complex64 fused k-qubit pass)"),
         "achieved": achieved,
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
@@ -425,12 +433,14 @@ def main():
         "launches": f_n,
         "avg_launch_ms": avg_ms,
         "mfma_tflops": tflops,
-        "mfma_frac": tflops / FP32_MFMA_PEAK_TFLOPS,
+        # split mode: four bf16 products per complex-GEMM flop
+        "mfma_frac": (4.0 * tflops / BF16_MFMA_PEAK_TFLOPS if args.mode == "split"
+                      else tflops / FP32_MFMA_PEAK_TFLOPS),
     }
     traffic_file = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(traffic_file):
         with open(traffic_file) as fh:
-            t = json.load(fh).get("k_tile_program" if args.mode == "tiles" else "k_fused_mfma")
+            t = json.load(fh).get("k_tile_program" if tiled else "k_fused_mfma")
         if t:
             result["roofline"]["traffic"] = t.get("bytes_per_launch")
 

@@ -97,6 +97,13 @@ void qc_swap(t_q_circuit *circuit, int qubit_a, int qubit_b);
 #define QCS_FUSE_TILES 0x10     /* fused form: tile programs -- several
                                    fused blocks per HBM pass (num_qubits >=
                                    16); default: one HBM pass per block */
+#define QCS_FUSE_SPLIT_BF16 0x40 /* fused form: tile programs (as
+                                    QCS_FUSE_TILES) whose blocks run on the
+                                    bf16 matrix cores as split products --
+                                    the FP32 matrix in three bf16 parts, the
+                                    state in two, FP32 accumulation -- so the
+                                    state is rounded to 16 significant bits
+                                    per block instead of 24 */
 #define QCS_FUSE_K(k) ((k) << 8) /* fused form: max qubits per fused block,
                                     1..5 (default 5; at most 4 with
                                     QCS_FUSE_TILES) */

@@ -184,12 +184,14 @@ int qcs_hip_apply_unitary(qcs_hip_state *s, int k, const int *targets,
  * applied to every tile in LDS before the tile is written back.  complex64
  * states with num_qubits >= tile_bits + 3; tile_bits is 13; blocks of a
  * pass act on at most QCS_HIP_TILE_KMAX qubits and a pass holds at most
- * QCS_HIP_TILE_MAX_BLOCKS blocks (their operands stay in LDS).  No reference
- * counterpart. */
+ * QCS_HIP_TILE_MAX_BLOCKS blocks (their operands stay in LDS;
+ * QCS_HIP_TILE_MAX_BLOCKS_SPLIT with QCS_HIP_TILE_MATH_SPLIT_BF16).  No
+ * reference counterpart. */
 #define QCS_HIP_TILE_BITS 13
 #define QCS_HIP_TILE_LOW_BITS 4
 #define QCS_HIP_TILE_KMAX 4
-#define QCS_HIP_TILE_MAX_BLOCKS 23
+#define QCS_HIP_TILE_MAX_BLOCKS 22
+#define QCS_HIP_TILE_MAX_BLOCKS_SPLIT 15
 typedef struct qcs_hip_pass {
   int tile_bits;
   int first_block;
@@ -209,6 +211,15 @@ int qcs_hip_apply_passes(qcs_hip_state *s, const qcs_hip_pass *passes,
  * 256; doubled when two fit). */
 #define QCS_HIP_TUNE_TILE_THREADS 0
 #define QCS_HIP_TUNE_TILE_GRID 1
+/* QCS_HIP_TUNE_TILE_MATH: arithmetic of the tile-program blocks, one of: */
+#define QCS_HIP_TUNE_TILE_MATH 2
+#define QCS_HIP_TILE_MATH_FP32 0       /* FP32 matrix-core GEMM (default) */
+#define QCS_HIP_TILE_MATH_SPLIT_BF16 1 /* the FP32 matrix as three bf16 parts,
+                                          each B value as two, summed on the
+                                          bf16 matrix cores with FP32
+                                          accumulation: the state is rounded
+                                          to 16 significant bits per block
+                                          (1 - F per block ~1e-11) */
 /* QCS_HIP_TUNE_BLOCK_VARIANT(k): launch variant of the one-block-per-pass
  * matrix-core kernel for blocks of k qubits (1..5), one of: */
 #define QCS_HIP_TUNE_BLOCK_VARIANT(k) (8 + (k))

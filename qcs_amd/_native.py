@@ -51,8 +51,10 @@ class HipPass(ctypes.Structure):
 
 
 TILE_BITS, TILE_LOW_BITS = 13, 4
-TUNE_TILE_THREADS, TUNE_TILE_GRID = 0, 1
+TUNE_TILE_THREADS, TUNE_TILE_GRID, TUNE_TILE_MATH = 0, 1, 2
+TILE_MATH_FP32, TILE_MATH_SPLIT_BF16 = 0, 1
 FUSE_TILES = 0x10
+FUSE_SPLIT_BF16 = 0x40
 DEFERRED = 0x20
 
 # batched circuits (qcs_hip.h / qcs.h)

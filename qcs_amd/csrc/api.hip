@@ -526,9 +526,9 @@ int qcs_hip_apply_passes(qcs_hip_state *s, const qcs_hip_pass *passes,
     if (ps.first_block != covered || ps.num_blocks < 0 ||
         ps.first_block + ps.num_blocks > num_blocks)
       return fail(QCS_HIP_EINVAL, "pass %d: blocks must follow the previous pass", p);
-    if (ps.num_blocks > tile_max_blocks())
+    if (ps.num_blocks > tile_max_blocks(s))
       return fail(QCS_HIP_EINVAL, "pass %d: %d blocks (at most %d)", p,
-                  ps.num_blocks, tile_max_blocks());
+                  ps.num_blocks, tile_max_blocks(s));
     for (int b = ps.first_block; b < ps.first_block + ps.num_blocks; ++b) {
       if (blocks[b].k > QCS_HIP_TILE_KMAX)
         return fail(QCS_HIP_EINVAL, "pass %d: block %d has k=%d (at most %d)", p,
@@ -561,6 +561,12 @@ int qcs_hip_set_tuning(qcs_hip_state *s, int knob, int value) {
     if (value < QCS_HIP_BLOCK_TB12_256 || value > QCS_HIP_BLOCK_TB13_512_PIPE)
       return fail(QCS_HIP_EINVAL, "unknown block variant %d", value);
     s->block_variant[knob - QCS_HIP_TUNE_BLOCK_VARIANT(0)] = value;
+    return 0;
+  }
+  if (knob == QCS_HIP_TUNE_TILE_MATH) {
+    if (value != QCS_HIP_TILE_MATH_FP32 && value != QCS_HIP_TILE_MATH_SPLIT_BF16)
+      return fail(QCS_HIP_EINVAL, "unknown tile math %d", value);
+    s->tile_math = value;
     return 0;
   }
   if (knob == QCS_HIP_TUNE_TILE_GRID) {

@@ -72,6 +72,7 @@ struct qcs_hip_state {
       QCS_HIP_BLOCK_TB13_512_PIPE, QCS_HIP_BLOCK_TB13_512_PIPE};
   int tile_threads = 1024;  // work-group size of the tile-program kernel
   int tile_grid = 256;      // persistent grid of the tile-program kernel
+  int tile_math = QCS_HIP_TILE_MATH_FP32;  // block arithmetic of tile programs
   // kernel timing
   int timing = 0;
   std::vector<qcs::TimedLaunch> pending_events;
@@ -148,7 +149,7 @@ int launch_compose(hipStream_t stream, const qcs_hip_gate *gates,
                    double *ubuf);
 
 // kernels in tiles.hip
-int tile_max_blocks();
+int tile_max_blocks(const qcs_hip_state *s);
 
 // kernels in blockpass.hip
 int launch_block_mfma(qcs_hip_state *s, int k, const int *targets,

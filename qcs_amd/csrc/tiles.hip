@@ -1,4 +1,4 @@
-// Tile programs: several fused blocks per HBM pass (complex64, FP32 MFMA).
+// Tile programs: several fused blocks per HBM pass (complex64, matrix cores).
 //
 // A pass owns a set S of TB = 13 index bits (ascending; always the lowest 4,
 // so a tile is read as >= 128-byte contiguous runs).  The state splits into
@@ -10,20 +10,47 @@
 // HBM.  HBM traffic per pass is one read and one write of the state, however
 // many blocks it holds.
 //
-// LDS layout.  The tile is two planes of 8192 floats, real parts then
-// imaginary parts; tile element e (bit j of e = index bit S[j]) sits at slot
-// swz(e) = e ^ (e>>5 & 31) ^ (e>>10 & 7) of both.  swz is linear over GF(2),
-// so the slot of (column c, row s) of a block is swz(col) ^ swz(row): the
-// host precomputes the 16 row terms of every block and each B-operand read
-// or result write costs one XOR.  The fold puts index bits 5..12 onto the
-// low slot bits, so the 32 columns a half-wave reads (the 5 lowest
-// non-target tile bits) are 32 distinct slots mod 32; in separate planes
-// those are 32 distinct 4-byte banks (interleaved {re, im} would put a
-// half-wave's 32 real parts 8 bytes apart, two per bank).  After the tile
-// come the A operands of every block of the pass, staged once per
-// work-group in lane order; they stay in LDS because on gfx9 every vector
-// memory load retires through one in-order counter, so a global load inside
-// the tile loop would wait behind the next tile's prefetch.
+// LDS layout.  The tile is two planes of 8192 floats, real parts at byte 0
+// and imaginary parts at PLANE = 32 KiB; tile element e (bit j of e = index
+// bit S[j]) sits at slot swz(e) = e ^ (e>>5 & 31) ^ (e>>10 & 7) of both.
+// swz is linear over GF(2), so the byte address of (column c, row s) of a
+// block is 4 swz(c) ^ 4 swz(row bits of s): the host precomputes the row
+// terms of every block, and since the kernel has no static LDS (checked on
+// the host) its dynamic LDS starts at address 0, so each B-operand read or
+// result write costs one XOR and no base add.  The fold puts index bits
+// 5..12 onto the low slot bits, so the 32 columns a half-wave reads (the 5
+// lowest non-target tile bits) are 32 distinct slots mod 32; in separate
+// planes those are 32 distinct 4-byte banks.
+//
+// After the tile come, per block of the pass, its A operands in lane order
+// and a 256-byte table: the per-lane part of the column address (column l32
+// with zero bits at the targets, through swz) and the byte offsets of the
+// lane's output rows.  Column c of a wave's group splits into that lane part
+// and a wave-uniform part computed on the scalar unit.  The operands stay
+// in LDS because on gfx9 every vector memory load retires through one
+// in-order counter, so a global load inside the tile loop would wait behind
+// the next tile's prefetch.
+//
+// Vector ALU work does not run alongside FP32 MFMA on gfx950 (their times
+// add; bf16 MFMA overlaps it only in part), so the block code is written for
+// few VALU instructions: table lookups instead of index arithmetic, uniform
+// values in scalar registers, one XOR per LDS access.  A column group's
+// B-operand reads are all issued before its MFMA chain: left to itself the
+// compiler reuses one register for every operand, and each MFMA then waits
+// a full LDS round trip.
+//
+// Split-bf16 math (QCS_HIP_TILE_MATH_SPLIT_BF16).  gfx950's bf16 matrix rate
+// is 16x its FP32 rate, so a block can instead run as a sum of bf16 GEMMs
+// (v_mfma_f32_32x32x16_bf16, FP32 accumulation): the FP32 matrix is split
+// into three bf16 parts W = Wh + Wm + Wl (exact: 3 x 8 significant bits),
+// each B value into two, x = xh + xl (|x - xh - xl| <= 2^-16 |x|), and
+//   W x ~ Wl xh + Wm xh + Wh xl + Wh xh
+// leaves out Wm xl and Wl xl (<= 2^-16 |W||x|).  The matrix is carried at
+// FP32 accuracy, so its error cannot add up coherently over repeated blocks;
+// what remains is a rounding of the state to 16 bits per block, which adds
+// up like a random walk.  The A operands of a block are staged as three
+// bf16 parts in lane order (6 KiB per block), so a pass holds at most
+// QCS_HIP_TILE_MAX_BLOCKS_SPLIT blocks.
 //
 // Work-group tile order is XCD aware: the hardware places work-group i on
 // XCD i % 8, so XCD x is given the contiguous x-th eighth of the tiles.
@@ -36,18 +63,31 @@ namespace qcs {
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef float f4v __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kPassMaxBits = 14;
 constexpr int kTileKMax = QCS_HIP_TILE_KMAX;  // qubits per block in a pass
 constexpr int kTileRMax = 1 << kTileKMax;
 constexpr int kTileWFloats = 64 * 64;          // W slot per block in wbuf
 constexpr int kTileBitsFixed = QCS_HIP_TILE_BITS;
+constexpr int kTilePlane = 4 << kTileBitsFixed;  // bytes per plane of the tile
 constexpr size_t kTileLdsTile = sizeof(float2) << kTileBitsFixed;
-constexpr size_t kTileLdsPerBlock = sizeof(float) * kTileRMax * 64;
 constexpr size_t kLdsPerCu = 160 * 1024;
-constexpr int kTileMaxBlocks = int((kLdsPerCu - kTileLdsTile) / kTileLdsPerBlock);
-static_assert(kTileMaxBlocks >= QCS_HIP_TILE_MAX_BLOCKS,
+// per block: A operands, then the lane table
+constexpr int kTileAFp32 = 4 * 64 * 16;             // [4][lane] float4
+constexpr int kTileASplit = 3 * 2 * 64 * 16;        // [part][instr][lane] 8 x bf16
+constexpr int kTileAux = 256;                        // lanecol[32], outoff[2][8]
+constexpr int kTileStrideFp32 = kTileAFp32 + kTileAux;
+constexpr int kTileStrideSplit = kTileASplit + kTileAux;
+static_assert(kTileLdsTile + size_t(kTileStrideFp32) * QCS_HIP_TILE_MAX_BLOCKS <= kLdsPerCu,
               "QCS_HIP_TILE_MAX_BLOCKS blocks must fit in LDS beside a tile");
+static_assert(kTileLdsTile + size_t(kTileStrideSplit) * QCS_HIP_TILE_MAX_BLOCKS_SPLIT <=
+                  kLdsPerCu,
+              "QCS_HIP_TILE_MAX_BLOCKS_SPLIT blocks must fit in LDS beside a tile");
 
 struct TilePass {
   int tb;
@@ -60,8 +100,10 @@ struct TilePass {
 struct TileBlock {
   int k;
   int wslot;
-  int tpos[kTileKMax];        // tile-bit positions of the targets, ascending
-  int rowslot[kTileRMax];     // slot swz(row bits of s) of row s
+  int tpos[kTileKMax];       // tile-bit positions of the targets, ascending
+  int rowoff[kTileRMax];     // 4 swz(row bits of s): byte offset of row s
+  int lanecol[32];           // 4 swz(l32 with zero bits at the targets)
+  int outoff[2][8];          // [h][2q + j]: rowoff of output row 4q + 2h + j
 };
 
 __host__ __device__ __forceinline__ int swz(int e) {
@@ -79,76 +121,194 @@ __device__ __forceinline__ void st_nt(float2 *p, float4 x) {
 }
 
 // insert a zero bit at position b (32-bit)
-__device__ __forceinline__ int ins0(int x, int b) {
+__host__ __device__ __forceinline__ int ins0(int x, int b) {
   return ((x >> b) << (b + 1)) | (x & ((1 << b) - 1));
 }
 
-// One block of a pass on the tile, its width K known at compile time.  The
-// B operand rows and the target positions are uniform (scalar loads); the
-// output rows of a lane depend on its half-wave, so their slots are computed
-// here once per block instead of being read per lane from global memory --
-// a vector load inside the tile loop would wait behind the next tile's
-// prefetch, which shares the in-order vmcnt counter.
+// LDS by absolute byte address (the dynamic LDS starts at 0)
+template <typename T>
+__device__ __forceinline__ T lds_load(int addr) {
+  typedef __attribute__((address_space(3))) T lds_t;
+  return *reinterpret_cast<lds_t *>(static_cast<size_t>(static_cast<unsigned>(addr)));
+}
+template <typename T>
+__device__ __forceinline__ void lds_store(int addr, T v) {
+  typedef __attribute__((address_space(3))) T lds_t;
+  *reinterpret_cast<lds_t *>(static_cast<size_t>(static_cast<unsigned>(addr))) = v;
+}
+
+// x0, x1 -> packed bf16 pairs hi (round to nearest even) and lo (the rounded
+// remainder): x = hi + lo within 2^-16 |x|
+__device__ __forceinline__ void split_pair(float x0, float x1, unsigned &hi,
+                                           unsigned &lo) {
+  const f2v x = {x0, x1};
+  const unsigned h = __builtin_bit_cast(unsigned, __builtin_convertvector(x, bf16x2));
+  const f2v r = {x0 - __uint_as_float(h << 16), x1 - __uint_as_float(h & 0xffff0000u)};
+  hi = h;
+  lo = __builtin_bit_cast(unsigned, __builtin_convertvector(r, bf16x2));
+}
+
+// What every block kernel needs of its block: uniform row offsets and target
+// positions (scalar registers), the lane's column part and output offsets.
+template <int K>
+struct BlockGeom {
+  static constexpr int R = 1 << K;
+  int tpos[K];
+  int ro[R];
+  int lp, lph;  // lane part of the column address: real plane, B plane
+  int os[8];    // output row offsets of this lane
+  __device__ __forceinline__ BlockGeom(const TileBlock *__restrict__ bl, int aux) {
+    const int lane = threadIdx.x & 63, h = lane >> 5, l32 = lane & 31;
+#pragma unroll
+    for (int q = 0; q < K; ++q) tpos[q] = bl->tpos[q];
+#pragma unroll
+    for (int s = 0; s < R; ++s) ro[s] = bl->rowoff[s];
+    lp = lds_load<int>(aux + 4 * l32);
+    lph = lp | (h ? kTilePlane : 0);
+    const i32x4 o0 = lds_load<i32x4>(aux + 128 + 32 * h);
+    const i32x4 o1 = lds_load<i32x4>(aux + 128 + 32 * h + 16);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      os[j] = o0[j];
+      os[4 + j] = o1[j];
+    }
+  }
+  // wave-uniform part of the address of column group `grp` (32 columns)
+  __device__ __forceinline__ int group_part(int grp) const {
+    int up = grp * 32;
+#pragma unroll
+    for (int q = 0; q < K; ++q) up = ins0(up, tpos[q]);
+    return 4 * swz(up);
+  }
+  // result rows of a 32x32 accumulator tile (rows 4q + 2h + j of the block)
+  __device__ __forceinline__ void store(int cb, const floatx16 &acc) const {
+    const int h = (threadIdx.x >> 5) & 1;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (4 * q >= R) continue;
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        if (4 * q + 2 * h + j < R) {
+          const int o = cb ^ os[2 * q + j];
+          lds_store<float>(o, acc[4 * q + 2 * j]);
+          lds_store<float>(o + kTilePlane, acc[4 * q + 2 * j + 1]);
+        }
+      }
+    }
+  }
+};
+
+// One block in FP32: lane (l32, h) supplies A[l32][2s + h] (output row
+// l32/2, part l32&1; input row s, part h) and B[2s + h][column l32] -- the
+// lane's column of plane h, row s -- for s = 0 .. R-1, one MFMA each.
 template <int K, int NTH, int TILE>
-__device__ __forceinline__ void tp_block(char *ldsb, const TileBlock *__restrict__ bl,
-                                         const float *wlb) {
-  constexpr int R = 1 << K, NW = NTH / 64, NCOLS = TILE >> K;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int h = lane >> 5, l32 = lane & 31;
-  float *re = reinterpret_cast<float *>(ldsb);
-  float *im = re + TILE;
-  const float *bplane = h ? im : re;  // the lane's part of the B operand
+__device__ __forceinline__ void tp_block(const TileBlock *__restrict__ bl, int blk) {
+  constexpr int R = 1 << K, NW = NTH / 64, GROUPS = (TILE >> K) / 32 / NW;
+  static_assert(GROUPS >= 1, "a wave takes whole column groups");
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   float wa[R];
 #pragma unroll
-  for (int s = 0; s < R; ++s) wa[s] = wlb[s * 64 + lane];
-  int tpos[K], rs[R];
-#pragma unroll
-  for (int q = 0; q < K; ++q) tpos[q] = bl->tpos[q];
-#pragma unroll
-  for (int s = 0; s < R; ++s) rs[s] = bl->rowslot[s];
-  // output rows of this lane: accumulators 4q, 4q+1 hold row 4q + 2h,
-  // 4q+2, 4q+3 row 4q + 2h + 1; both candidates are uniform values
-  int os0[4], os1[4];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int a0 = rs[4 * q < R ? 4 * q : 0], a1 = rs[4 * q + 1 < R ? 4 * q + 1 : 0];
-    const int b0 = rs[4 * q + 2 < R ? 4 * q + 2 : 0], b1 = rs[4 * q + 3 < R ? 4 * q + 3 : 0];
-    os0[q] = __builtin_amdgcn_readfirstlane(a0) ^ (h ? (b0 ^ a0) : 0);
-    os1[q] = __builtin_amdgcn_readfirstlane(a1) ^ (h ? (b1 ^ a1) : 0);
+  for (int v = 0; v < (R + 3) / 4; ++v) {
+    const f4v x = lds_load<f4v>(blk + (v * 64 + lane) * 16);
+    wa[4 * v] = x.x;
+    if (4 * v + 1 < R) wa[4 * v + 1] = x.y;
+    if (4 * v + 2 < R) wa[4 * v + 2] = x.z;
+    if (4 * v + 3 < R) wa[4 * v + 3] = x.w;
   }
-  for (int c0 = wave * 32; c0 < NCOLS; c0 += NW * 32) {
-    int col = c0 + l32;
-#pragma unroll
-    for (int q = 0; q < K; ++q) col = ins0(col, tpos[q]);
-    const int cs = swz(col);  // slot of (col, row 0)
+  const BlockGeom<K> geo(bl, blk + kTileAFp32);
+#pragma unroll 1
+  for (int g = 0; g < GROUPS; ++g) {
+    const int up = geo.group_part(wave + g * NW);
+    const int cbh = geo.lph ^ up;
     float bv[R];
 #pragma unroll
-    for (int s = 0; s < R; ++s) bv[s] = bplane[cs ^ rs[s]];
+    for (int s = 0; s < R; ++s) bv[s] = lds_load<float>(cbh ^ geo.ro[s]);
+    // all reads first, one wait, then the chain: interleaved, each MFMA
+    // would wait out an LDS round trip for its operand
+    __builtin_amdgcn_sched_barrier(0);
     floatx16 acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f,
                     0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int s = 0; s < R; ++s)
       acc = __builtin_amdgcn_mfma_f32_32x32x2f32(wa[s], bv[s], acc, 0, 0, 0);
+    geo.store(geo.lp ^ up, acc);
+  }
+}
+
+// One block in split-bf16 form (see the file comment).  Lane (l32, h)
+// supplies K-index 8h + j of instruction c, which is input row s = 8c + j,
+// part h of the B operand -- the same rows and plane as the FP32 form reads
+// -- and the output rows are those of the FP32 form.  A wave takes G column
+// groups at once: G independent accumulation chains, so one chain's split
+// overlaps another's matrix-core work.
+template <int K, int NTH, int TILE>
+__device__ __forceinline__ void tp_block_split(const TileBlock *__restrict__ bl, int blk) {
+  constexpr int R = 1 << K, NC = R > 8 ? 2 : 1, NW = NTH / 64;
+  constexpr int GROUPS = (TILE >> K) / 32 / NW;
+  constexpr int G = GROUPS >= 2 ? 2 : 1;
+  static_assert(GROUPS >= 1 && GROUPS % G == 0, "column groups must split evenly");
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  bf16x8 wh[NC], wm[NC], wl[NC];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      if (4 * q >= R) continue;
-      const int r0 = 4 * q + 2 * h;
-      if (r0 < R) {
-        re[cs ^ os0[q]] = acc[4 * q];
-        im[cs ^ os0[q]] = acc[4 * q + 1];
+  for (int c = 0; c < NC; ++c) {
+    wh[c] = lds_load<bf16x8>(blk + ((0 * 2 + c) * 64 + lane) * 16);
+    wm[c] = lds_load<bf16x8>(blk + ((1 * 2 + c) * 64 + lane) * 16);
+    wl[c] = lds_load<bf16x8>(blk + ((2 * 2 + c) * 64 + lane) * 16);
+  }
+  const BlockGeom<K> geo(bl, blk + kTileASplit);
+#pragma unroll 1
+  for (int g0 = 0; g0 < GROUPS; g0 += G) {
+    int up[G];
+    float bv[G][8 * NC];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      up[g] = geo.group_part(wave + (g0 + g) * NW);
+      const int cbh = geo.lph ^ up[g];
+#pragma unroll
+      for (int s = 0; s < 8 * NC; ++s)
+        bv[g][s] = s < R ? lds_load<float>(cbh ^ geo.ro[s < R ? s : 0]) : 0.0f;
+    }
+    __builtin_amdgcn_sched_barrier(0);  // all B reads before the splits
+    floatx16 acc[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+      acc[g] = floatx16{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f,
+                        0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      bf16x8 xh[G], xl[G];
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        u32x4 hi, lo;
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+          unsigned a, b;
+          split_pair(bv[g][8 * c + 2 * p], bv[g][8 * c + 2 * p + 1], a, b);
+          hi[p] = a;
+          lo[p] = b;
+        }
+        xh[g] = __builtin_bit_cast(bf16x8, hi);
+        xl[g] = __builtin_bit_cast(bf16x8, lo);
       }
-      if (r0 + 1 < R) {
-        re[cs ^ os1[q]] = acc[4 * q + 2];
-        im[cs ^ os1[q]] = acc[4 * q + 3];
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        acc[g] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wl[c], xh[g], acc[g], 0, 0, 0);
+        acc[g] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wm[c], xh[g], acc[g], 0, 0, 0);
+        acc[g] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wh[c], xl[g], acc[g], 0, 0, 0);
+        acc[g] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wh[c], xh[g], acc[g], 0, 0, 0);
       }
     }
+#pragma unroll
+    for (int g = 0; g < G; ++g) geo.store(geo.lp ^ up[g], acc[g]);
   }
 }
 
 // The descriptor pointers are __restrict__ so the compiler may read them
 // through the scalar cache (they are uniform and never written by the
 // kernel); vector loads of them would wait behind the tile prefetch.
-template <int TB, int NTH>
+template <int TB, int NTH, bool SPLIT>
 __global__ __launch_bounds__(NTH) void k_tile_program(
     float2 *__restrict__ a, const TilePass *__restrict__ passes, int pass_index,
     const TileBlock *__restrict__ blocks, const float *__restrict__ wbuf,
@@ -157,25 +317,47 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
   constexpr int TBITS = __builtin_ctz(NTH) + 1;  // tile bits covered by 2*tid
   constexpr int IT = TILE / (2 * NTH);           // float4 loads per thread
   constexpr int IB = TB - TBITS;                 // tile bits covered by `it`
-  constexpr int RM = kTileRMax;
+  constexpr int STRIDE = SPLIT ? kTileStrideSplit : kTileStrideFp32;
+  constexpr int ABYTES = SPLIT ? kTileASplit : kTileAFp32;
+  constexpr int OPS = int(sizeof(float2)) << TB;  // first block's LDS address
+  static_assert(TB == kTileBitsFixed, "the plane offset assumes 13-bit tiles");
   extern __shared__ float4 smem[];
-  char *ldsb = reinterpret_cast<char *>(smem);
-  float *wl = reinterpret_cast<float *>(ldsb + (sizeof(float2) << TB));
+  float *re = reinterpret_cast<float *>(smem);
+  float *im = re + TILE;
 
   const TilePass &ps = passes[pass_index];
   const int nb = ps.nblocks;
   const int tid = threadIdx.x;
 
-  // stage the A operands of every block: wl[(b*RM + s)*64 + lane] holds
-  // W_b[l32][2s + h] (zero outside the block's 2^(k+1) x 2^(k+1) matrix)
-  for (int idx = tid; idx < nb * RM * 64; idx += NTH) {
-    const int b = idx / (RM * 64), rem = idx % (RM * 64);
-    const int s = rem >> 6, ln = rem & 63;
+  // stage the A operands and lane tables of every block
+  for (int idx = tid; idx < nb * 64 * 16; idx += NTH) {
+    const int b = idx / (64 * 16), ln = (idx / 16) % 64, s = idx % 16;
     const TileBlock &bl = blocks[ps.first_block + b];
     const int R = 1 << bl.k, RR = 2 * R, row = ln & 31, hh = ln >> 5;
-    wl[idx] = (row < RR && s < R)
-                  ? wbuf[size_t(bl.wslot) * kTileWFloats + row * RR + 2 * s + hh]
-                  : 0.0f;
+    const float w = (row < RR && s < R)
+                        ? wbuf[size_t(bl.wslot) * kTileWFloats + row * RR + 2 * s + hh]
+                        : 0.0f;
+    const int blk = OPS + b * STRIDE;
+    if (!SPLIT) {
+      // [s / 4][lane] float4, element s % 4
+      lds_store<float>(blk + ((s >> 2) * 64 + ln) * 16 + (s & 3) * 4, w);
+    } else {
+      // part P of W[l32][2s + h], s = 8c + j: [P][c][lane] x 8 bf16
+      const __bf16 ph = __bf16(w);
+      const float r1 = w - float(ph);
+      const __bf16 pm = __bf16(r1);
+      const __bf16 pl = __bf16(r1 - float(pm));
+      const int c = s >> 3, j = s & 7;
+      lds_store<__bf16>(blk + ((0 * 2 + c) * 64 + ln) * 16 + 2 * j, ph);
+      lds_store<__bf16>(blk + ((1 * 2 + c) * 64 + ln) * 16 + 2 * j, pm);
+      lds_store<__bf16>(blk + ((2 * 2 + c) * 64 + ln) * 16 + 2 * j, pl);
+    }
+  }
+  for (int idx = tid; idx < nb * 48; idx += NTH) {
+    const int b = idx / 48, e = idx % 48;
+    const TileBlock &bl = blocks[ps.first_block + b];
+    const int v = e < 32 ? bl.lanecol[e] : bl.outoff[(e - 32) >> 3][(e - 32) & 7];
+    lds_store<int>(OPS + b * STRIDE + ABYTES + 4 * e, v);
   }
 
   // element e = 2*tid + 2*NTH*it and its neighbour e+1 (tile bit 0)
@@ -207,25 +389,32 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
     return b;
   };
 
+  // The next tile is loaded into registers while the current one is
+  // computed.  At a tile boundary the finished tile is read out of LDS, the
+  // prefetched one written in, and only then are the stores issued: stores
+  // count against the same in-order vector-memory counter as loads, so a
+  // wait for the prefetch placed after the stores would also wait for the
+  // stores to reach memory.
   float4 v[IT];
   uint64_t base = tile_base(tile);
   if (tile < tend) {
 #pragma unroll
     for (int it = 0; it < IT; ++it) v[it] = ld_nt(a + (base | it_addr(it) | addr_t));
   }
-  __syncthreads();  // A operands staged
-
-  for (; tile < tend; tile += G8) {
+  __syncthreads();  // operands staged
+  if (tile < tend) {
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
       const int s0 = slot_t ^ swz(it << TBITS);  // slot of e; e+1 is s0 ^ 1
-      float *re = reinterpret_cast<float *>(ldsb), *im = re + TILE;
       re[s0] = v[it].x;
       im[s0] = v[it].y;
       re[s0 ^ 1] = v[it].z;
       im[s0 ^ 1] = v[it].w;
     }
-    __syncthreads();
+  }
+  __syncthreads();
+
+  for (; tile < tend; tile += G8) {
     const uint64_t next = tile + G8;
     const uint64_t nbase = tile_base(next);
     if (next < tend) {
@@ -236,25 +425,46 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
 
     for (int b = 0; b < nb; ++b) {
       const TileBlock *bl = &blocks[ps.first_block + b];
-      const float *wlb = wl + b * RM * 64;
-      switch (bl->k) {
-        case 1: tp_block<1, NTH, TILE>(ldsb, bl, wlb); break;
-        case 2: tp_block<2, NTH, TILE>(ldsb, bl, wlb); break;
-        case 3: tp_block<3, NTH, TILE>(ldsb, bl, wlb); break;
-        default: tp_block<4, NTH, TILE>(ldsb, bl, wlb); break;
+      const int blk = OPS + b * STRIDE;
+      if (SPLIT) {
+        switch (bl->k) {
+          case 1: tp_block_split<1, NTH, TILE>(bl, blk); break;
+          case 2: tp_block_split<2, NTH, TILE>(bl, blk); break;
+          case 3: tp_block_split<3, NTH, TILE>(bl, blk); break;
+          default: tp_block_split<4, NTH, TILE>(bl, blk); break;
+        }
+      } else {
+        switch (bl->k) {
+          case 1: tp_block<1, NTH, TILE>(bl, blk); break;
+          case 2: tp_block<2, NTH, TILE>(bl, blk); break;
+          case 3: tp_block<3, NTH, TILE>(bl, blk); break;
+          default: tp_block<4, NTH, TILE>(bl, blk); break;
+        }
       }
       __syncthreads();
     }
 
+    float4 o[IT];
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
       const int s0 = slot_t ^ swz(it << TBITS);
-      const float *re = reinterpret_cast<const float *>(ldsb), *im = re + TILE;
-      const float2 x = make_float2(re[s0], im[s0]);
-      const float2 y = make_float2(re[s0 ^ 1], im[s0 ^ 1]);
-      st_nt(a + (base | it_addr(it) | addr_t), make_float4(x.x, x.y, y.x, y.y));
+      o[it] = make_float4(re[s0], im[s0], re[s0 ^ 1], im[s0 ^ 1]);
     }
-    __syncthreads();
+    __syncthreads();  // the finished tile is out of LDS
+    if (next < tend) {
+#pragma unroll
+      for (int it = 0; it < IT; ++it) {
+        const int s0 = slot_t ^ swz(it << TBITS);
+        re[s0] = v[it].x;
+        im[s0] = v[it].y;
+        re[s0 ^ 1] = v[it].z;
+        im[s0 ^ 1] = v[it].w;
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);  // keep the stores after the LDS writes
+#pragma unroll
+    for (int it = 0; it < IT; ++it) st_nt(a + (base | it_addr(it) | addr_t), o[it]);
+    __syncthreads();  // the next tile is in LDS
     base = nbase;
   }
 }
@@ -271,27 +481,69 @@ static int grow_dev(void **p, size_t *cap, size_t need) {
   return 0;
 }
 
-template <int NTH>
+template <int NTH, bool SPLIT>
 static int launch_tp(qcs_hip_state *s, const TilePass *dpass, int pi,
                      const TileBlock *dblocks, uint64_t ntiles, int nblocks) {
   static bool attr_set = false;
   if (!attr_set) {
-    QCS_HIP_TRY(hipFuncSetAttribute(
-        reinterpret_cast<const void *>(&k_tile_program<kTileBitsFixed, NTH>),
-        hipFuncAttributeMaxDynamicSharedMemorySize, int(kLdsPerCu)));
+    const void *fn = reinterpret_cast<const void *>(&k_tile_program<kTileBitsFixed, NTH, SPLIT>);
+    QCS_HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    int(kLdsPerCu)));
+    // LDS is addressed by absolute byte address: the dynamic LDS must start
+    // at 0, i.e. the kernel has no static LDS
+    hipFuncAttributes fa;
+    QCS_HIP_TRY(hipFuncGetAttributes(&fa, fn));
+    if (fa.sharedSizeBytes != 0)
+      return fail(QCS_HIP_EDEVICE, "k_tile_program has %zu bytes of static LDS",
+                  size_t(fa.sharedSizeBytes));
     attr_set = true;
   }
-  const size_t lds = kTileLdsTile + kTileLdsPerBlock * size_t(nblocks);
+  const size_t lds =
+      kTileLdsTile + size_t(SPLIT ? kTileStrideSplit : kTileStrideFp32) * size_t(nblocks);
   const uint64_t per_cu = kLdsPerCu / lds;  // work-groups that fit per CU
   uint64_t grid = uint64_t(s->tile_grid) * (per_cu > 1 ? 2 : 1);
   if (grid > ntiles) grid = ntiles;
-  hipLaunchKernelGGL((k_tile_program<kTileBitsFixed, NTH>), dim3(unsigned(grid)),
+  hipLaunchKernelGGL((k_tile_program<kTileBitsFixed, NTH, SPLIT>), dim3(unsigned(grid)),
                      dim3(NTH), lds, s->stream, static_cast<float2 *>(s->vec),
                      dpass, pi, dblocks, s->wbuf, ntiles);
   return launch_check("k_tile_program");
 }
 
-int tile_max_blocks() { return QCS_HIP_TILE_MAX_BLOCKS; }
+int tile_max_blocks(const qcs_hip_state *s) {
+  return s->tile_math == QCS_HIP_TILE_MATH_SPLIT_BF16 ? QCS_HIP_TILE_MAX_BLOCKS_SPLIT
+                                                      : QCS_HIP_TILE_MAX_BLOCKS;
+}
+
+// the block's descriptor: target positions, row offsets, lane tables
+static void make_tile_block(const qcs_hip_pass &ps, const qcs_hip_block &blk, int wslot,
+                            TileBlock *tb) {
+  std::memset(tb, 0, sizeof(TileBlock));
+  const int k = blk.k, R = 1 << k;
+  tb->k = k;
+  tb->wslot = wslot;
+  for (int q = 0; q < k; ++q) {
+    int pos = 0;
+    while (ps.bits[pos] != blk.targets[q]) ++pos;
+    tb->tpos[q] = pos;
+  }
+  for (int r = 0; r < R; ++r) {
+    int off = 0;
+    for (int q = 0; q < k; ++q)
+      if ((r >> q) & 1) off |= 1 << tb->tpos[q];
+    tb->rowoff[r] = 4 * swz(off);
+  }
+  for (int l = 0; l < 32; ++l) {
+    int c = l;
+    for (int q = 0; q < k; ++q) c = ins0(c, tb->tpos[q]);
+    tb->lanecol[l] = 4 * swz(c);
+  }
+  for (int h = 0; h < 2; ++h)
+    for (int q = 0; q < 4; ++q)
+      for (int j = 0; j < 2; ++j) {
+        const int r = 4 * q + 2 * h + j;
+        tb->outoff[h][2 * q + j] = r < R ? tb->rowoff[r] : 0;
+      }
+}
 
 int tiles_apply(qcs_hip_state *s, const qcs_hip_pass *passes, int np,
                 const qcs_hip_block *blocks, int nb, const qcs_hip_gate *gates,
@@ -314,22 +566,8 @@ int tiles_apply(qcs_hip_state *s, const qcs_hip_pass *passes, int np,
     tp[p].first_block = passes[p].first_block;
     for (int j = 0; j < passes[p].tile_bits; ++j) tp[p].sbits[j] = passes[p].bits[j];
     for (int b = passes[p].first_block;
-         b < passes[p].first_block + passes[p].num_blocks; ++b) {
-      std::memset(&tb[b], 0, sizeof(TileBlock));
-      tb[b].k = blocks[b].k;
-      tb[b].wslot = b;
-      for (int q = 0; q < blocks[b].k; ++q) {
-        int pos = 0;
-        while (passes[p].bits[pos] != blocks[b].targets[q]) ++pos;
-        tb[b].tpos[q] = pos;
-      }
-      for (int r = 0; r < kTileRMax; ++r) {
-        int off = 0;
-        for (int q = 0; q < blocks[b].k; ++q)
-          if ((r >> q) & 1) off |= 1 << tb[b].tpos[q];
-        tb[b].rowslot[r] = swz(off);
-      }
-    }
+         b < passes[p].first_block + passes[p].num_blocks; ++b)
+      make_tile_block(passes[p], blocks[b], b, &tb[b]);
   }
   QCS_HIP_TRY(hipMemcpyAsync(s->tpass_dev, tp, pbytes, hipMemcpyHostToDevice,
                              s->stream));
@@ -337,6 +575,7 @@ int tiles_apply(qcs_hip_state *s, const qcs_hip_pass *passes, int np,
                              s->stream));
   const TilePass *dp = static_cast<const TilePass *>(s->tpass_dev);
   const TileBlock *db = static_cast<const TileBlock *>(s->tblock_dev);
+  const bool split = s->tile_math == QCS_HIP_TILE_MATH_SPLIT_BF16;
   const double N = double(s->size);
   for (int p = 0; p < np; ++p) {
     const uint64_t ntiles = s->size >> passes[p].tile_bits;
@@ -346,10 +585,13 @@ int tiles_apply(qcs_hip_state *s, const qcs_hip_pass *passes, int np,
       flops += 8.0 * double(1 << blocks[b].k) * N;
     TimedLaunch tl;
     timing_begin(s, QCS_HIP_K_FUSED, 2.0 * N * double(s->amp_bytes), flops, &tl);
+    const int nbp = passes[p].num_blocks;
     if (s->tile_threads == 1024)
-      rc = launch_tp<1024>(s, dp, p, db, ntiles, passes[p].num_blocks);
+      rc = split ? launch_tp<1024, true>(s, dp, p, db, ntiles, nbp)
+                 : launch_tp<1024, false>(s, dp, p, db, ntiles, nbp);
     else
-      rc = launch_tp<512>(s, dp, p, db, ntiles, passes[p].num_blocks);
+      rc = split ? launch_tp<512, true>(s, dp, p, db, ntiles, nbp)
+                 : launch_tp<512, false>(s, dp, p, db, ntiles, nbp);
     if (rc) return rc;
     if ((rc = timing_end(s, &tl))) return rc;
   }

@@ -157,7 +157,7 @@ static void record(t_q_circuit *c, int kind, int target, int control,
  * states large enough for the tile kernel when QCS_FUSE_TILES asks for
  * them */
 static int use_tiles(const t_q_circuit *c) {
-  return is_fused(c) && (c->options & QCS_FUSE_TILES) &&
+  return is_fused(c) && (c->options & (QCS_FUSE_TILES | QCS_FUSE_SPLIT_BF16)) &&
          c->num_qubits >= QCS_HIP_TILE_BITS + 3;
 }
 
@@ -178,7 +178,10 @@ static void flush_pending(t_q_circuit *c) {
   if (use_tiles(c))
     rc = qcs_pass_plan(c->pending, c->pending_size, c->num_qubits,
                        QCS_HIP_TILE_BITS, QCS_HIP_TILE_LOW_BITS, fuse_k(c),
-                       QCS_HIP_TILE_MAX_BLOCKS, &passes, &np, &blocks, &nb,
+                       (c->options & QCS_FUSE_SPLIT_BF16)
+                           ? QCS_HIP_TILE_MAX_BLOCKS_SPLIT
+                           : QCS_HIP_TILE_MAX_BLOCKS,
+                       &passes, &np, &blocks, &nb,
                        &ordered);
   else
     rc = qcs_fusion_plan(c->pending, c->pending_size, c->num_qubits,
@@ -251,6 +254,10 @@ t_q_circuit *qc_create_ex(int num_qubits, int options) {
     fprintf(stderr, "Error: QCS_REFERENCE_COMPAT applies to the plain forms only.\n");
     return NULL;
   }
+  if ((options & QCS_FUSE_SPLIT_BF16) && !(options & QCS_FUSED_FP32)) {
+    fprintf(stderr, "Error: QCS_FUSE_SPLIT_BF16 applies to the fused form only.\n");
+    return NULL;
+  }
   c = (t_q_circuit *)malloc(sizeof(t_q_circuit));
   if (!c) return NULL;
   memset(c, 0, sizeof(*c));
@@ -274,7 +281,8 @@ t_q_circuit *qc_create_ex(int num_qubits, int options) {
     return c;
   }
   if (options & QCS_DEFERRED) {
-    if ((options & (QCS_PLAIN_FP32 | QCS_REFERENCE_COMPAT | QCS_FUSE_TILES)) ||
+    if ((options & (QCS_PLAIN_FP32 | QCS_REFERENCE_COMPAT | QCS_FUSE_TILES |
+                    QCS_FUSE_SPLIT_BF16)) ||
         num_qubits > QCS_HIP_BATCH_MAX_QUBITS) {
       fprintf(stderr, "Error: QCS_DEFERRED takes 1..%d qubits, complex128 "
                       "plain or QCS_FUSED_FP32.\n", QCS_HIP_BATCH_MAX_QUBITS);
@@ -290,6 +298,13 @@ t_q_circuit *qc_create_ex(int num_qubits, int options) {
   if (rc != 0) {
     fprintf(stderr, "Error: could not create a %d-qubit device state: %s\n",
             num_qubits, qcs_hip_last_error());
+    qc_destroy(c);
+    return NULL;
+  }
+  if ((options & QCS_FUSE_SPLIT_BF16) &&
+      qcs_hip_set_tuning(c->state, QCS_HIP_TUNE_TILE_MATH,
+                         QCS_HIP_TILE_MATH_SPLIT_BF16) != 0) {
+    fprintf(stderr, "Error: %s\n", qcs_hip_last_error());
     qc_destroy(c);
     return NULL;
   }

@@ -146,7 +146,7 @@ int qc_autotune(t_q_circuit *c) {
   if (!c || !c->state || !(c->options & QCS_FUSED_FP32)) return QCS_HIP_EINVAL;
   n = c->num_qubits;
   kmax = n < QCS_HIP_MAX_FUSE ? n : QCS_HIP_MAX_FUSE;
-  if ((c->options & QCS_FUSE_TILES) && kmax > QCS_HIP_TILE_KMAX)
+  if ((c->options & (QCS_FUSE_TILES | QCS_FUSE_SPLIT_BF16)) && kmax > QCS_HIP_TILE_KMAX)
     kmax = QCS_HIP_TILE_KMAX;
   if ((rc = device_key(dev, sizeof(dev)))) return rc;
   for (k = 1; k <= kmax; k++) {

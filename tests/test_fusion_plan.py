@@ -103,7 +103,7 @@ def test_plan_state_level_12q():
     np.testing.assert_allclose(psi, HP.simulate(n, ops), atol=1e-12)
 
 
-def pass_plan(ops, n, tb=13, low=6, k=4, max_blocks=23):
+def pass_plan(ops, n, tb=13, low=6, k=4, max_blocks=22):
     _, qcs = qcs_amd.load()
     gates = to_gates(ops)
     pp = ctypes.POINTER(qcs_amd.HipPass)()

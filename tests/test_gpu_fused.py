@@ -200,6 +200,81 @@ def test_fused_tile_mode_layered_vs_plain(qlib, n, depth):
         assert f.get_num_fused_passes() < f.get_num_fused_blocks()
 
 
+@pytest.mark.parametrize("n,threads", [(16, 512), (17, 1024), (20, 1024)])
+def test_split_bf16_tile_programs(qlib, n, threads):
+    """Split-bf16 tile math applies the same blocks as the FP32 form within
+    the rounding of the state to 16 significant bits per block: 1 - F
+    against the FP64 simulation stays far below the form's 1e-6 (dropping a
+    split product, e.g. the matrix's middle part, costs ~2^-8 per block and
+    fails the 1e-8 bound here)."""
+    from test_fusion_plan import pass_plan
+    hip, _ = qlib
+    print(f"[fused] split-bf16 tile programs n={n} threads={threads}", flush=True)
+    kinds = [C.H, C.X, C.Y, C.Z, C.P, C.RX, C.RY, C.RZ, C.CNOT, C.CPHASE, C.SWAP]
+    ops = C.random_circuit(n, 300, 4242 + n, kinds=kinds)
+    passes, blocks, ordered = pass_plan(ops, n, low=4, max_blocks=15)
+    P = (qcs_amd.HipPass * len(passes))(*passes)
+    B = (qcs_amd.HipBlock * len(blocks))(*blocks)
+    G = (qcs_amd.HipGate * len(ordered))(*ordered)
+    psi = rand_state(n, np.random.default_rng(7 * n))
+    with qcs_amd.Circuit(n, qcs_amd.PLAIN_FP32) as a, \
+            qcs_amd.Circuit(n, qcs_amd.PLAIN_FP32) as b:
+        a.set_amplitudes(psi)
+        b.set_amplitudes(psi)
+        for st in (a, b):
+            assert hip.qcs_hip_set_tuning(st.device_state, qcs_amd.TUNE_TILE_THREADS,
+                                          threads) == 0
+        assert hip.qcs_hip_set_tuning(a.device_state, qcs_amd.TUNE_TILE_MATH,
+                                      qcs_amd.TILE_MATH_SPLIT_BF16) == 0
+        for st in (a, b):
+            rc = hip.qcs_hip_apply_passes(st.device_state, P, len(passes), B, len(blocks),
+                                          G, len(ordered))
+            assert rc == 0, hip.qcs_hip_last_error()
+        got, fp32 = a.amplitudes(), b.amplitudes()
+    ref = HP.simulate(n, ops, psi.astype(np.complex64).astype(complex))
+    omf = 1 - abs(np.vdot(ref, got)) ** 2 / (np.vdot(got, got).real * np.vdot(ref, ref).real)
+    omf32 = 1 - abs(np.vdot(ref, fp32)) ** 2 / (np.vdot(fp32, fp32).real
+                                                 * np.vdot(ref, ref).real)
+    print(f"[fused] split 1-F {omf:.3e} (FP32 {omf32:.3e}), {len(blocks)} blocks", flush=True)
+    assert omf <= 1e-8
+    assert np.max(np.abs(got - ref)) < 2e-5 * np.max(np.abs(ref)) * math.sqrt(len(blocks))
+
+
+def test_split_bf16_pass_limit_and_options(qlib):
+    from test_fusion_plan import pass_plan
+    hip, qcs = qlib
+    n = 16
+    ops = C.random_circuit(12, 800, 5, kinds=[C.H, C.RX, C.RY, C.CNOT])
+    passes, blocks, ordered = pass_plan(ops, n, low=4, max_blocks=22)
+    assert max(p.num_blocks for p in passes) > 15  # the plan needs the FP32 limit
+    P = (qcs_amd.HipPass * len(passes))(*passes)
+    B = (qcs_amd.HipBlock * len(blocks))(*blocks)
+    G = (qcs_amd.HipGate * len(ordered))(*ordered)
+    with qcs_amd.Circuit(n, qcs_amd.PLAIN_FP32) as a:
+        assert hip.qcs_hip_set_tuning(a.device_state, qcs_amd.TUNE_TILE_MATH, 7) == -1
+        assert hip.qcs_hip_set_tuning(a.device_state, qcs_amd.TUNE_TILE_MATH,
+                                      qcs_amd.TILE_MATH_SPLIT_BF16) == 0
+        assert hip.qcs_hip_apply_passes(a.device_state, P, len(passes), B, len(blocks),
+                                        G, len(ordered)) == -1
+    # the option belongs to the fused form
+    assert not qcs.qc_create_ex(n, qcs_amd.FUSE_SPLIT_BF16)
+    assert not qcs.qc_create_ex(n, qcs_amd.PLAIN_FP32 | qcs_amd.FUSE_SPLIT_BF16)
+
+
+@pytest.mark.parametrize("n,depth", [(16, 6), (20, 8), (24, 4)])
+def test_fused_split_mode_layered_vs_plain(qlib, n, depth):
+    print(f"[fused] split-bf16 mode layered n={n}", flush=True)
+    ops = C.layered(n, depth, 3)
+    with qcs_amd.Circuit(n) as ref, \
+            qcs_amd.Circuit(n, qcs_amd.FUSED_FP32 | qcs_amd.FUSE_SPLIT_BF16) as f:
+        ref.apply(ops)
+        f.apply(ops)
+        omf, mx = f.compare_with(ref)
+        print(f"[fused] split mode 1-F {omf:.3e} max-abs {mx:.3e}", flush=True)
+        assert omf <= 1e-8 and mx < 1e-5, (omf, mx)
+        assert f.get_num_fused_passes() < f.get_num_fused_blocks()
+
+
 def test_fused_qft_with_swaps_closed_form(qlib):
     n, b = 14, 9999
     with qcs_amd.Circuit(n, qcs_amd.FUSED_FP32) as f:
