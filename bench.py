@@ -5,16 +5,20 @@ One step = one full run of the 30-qubit depth-20 layered circuit (H on every
 qubit, then 20 layers of Rz-Ry-Rz on every qubit and a CNOT ladder; angles
 from splitmix64(seed 2); 2,410 gates) on a complex64 state resident in HBM:
 reset to |0...0>, record the gates through the qc_* API, flush (fusion plan,
-FP64 composition of the fused unitaries on the device, one FP32 matrix-core
-pass per fused block, FP64 renormalisation).  value = gates * 2^30 / step
-time.
+FP64 composition of the fused unitaries on the device, tile programs that
+apply several fused blocks per HBM pass on the FP32 matrix cores, FP64
+renormalisation).  value = gates * 2^30 / step time.
 
 Beside it, in the same process:
   * the plain FP64 form of the same circuit (complex128, one kernel per gate),
     for the speedup and for 1 - F / max-abs of the fused state;
   * the dominant kernel's roofline: average duration of the fused
     matrix-core kernel from HIP events recorded on the state's stream inside
-    the timed steps; algorithmic bytes per launch = 2 * 2^30 * 8;
+    the timed steps; tile programs are bound by the FP32 matrix cores
+    (algorithmic flops per launch = 8 * 2^k * 2^30 per block of the pass),
+    one-block passes by HBM (2 * 2^30 * 8 bytes per launch);
+  * the other fused modes on the same circuit (other_modes): one HBM pass
+    per fused block (k <= 5), and split-bf16 tile programs;
   * the CPU baseline: the reference's own sequential library (oracle/_ref,
     built from its sources) on the 20-qubit depth-10 circuit, first gates
     only (a bounded sample), 1 core; beside it the reference's OpenMP build
@@ -50,12 +54,14 @@ def parse():
     p.add_argument("--seed", type=int, default=2)
     p.add_argument("--fuse-k", type=int, default=0,
                    help="max qubits per fused block (default: 4 in tile programs, 5 per-block)")
-    p.add_argument("--mode", choices=["tiles", "split", "blocks"], default="blocks",
+    p.add_argument("--mode", choices=["tiles", "split", "blocks"], default="tiles",
                    help="tiles: several fused blocks per HBM pass; split: tiles whose "
                         "blocks run as split-bf16 products; blocks: one pass per block")
     p.add_argument("--tile-threads", type=int, default=1024)
     p.add_argument("--no-plain", action="store_true", help="skip the plain FP64 run")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--no-alt", action="store_true",
+                   help="skip the timing of the other fused modes")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--workload", choices=["fused30", "batch"], default="fused30",
                    help="fused30: the 30-qubit fused-vs-plain line (default); batch: "
@@ -317,6 +323,81 @@ def bench_batch(args):
     print(json.dumps(result), flush=True)
 
 
+def run_mode(hip, ops, n, mode, args, steps, warmup, tune):
+    """Time `steps` fused runs of `ops` in one fused mode; returns the open
+    circuit (its final state) and the figures."""
+    import qcs_amd
+    G, N = len(ops), float(1 << n)
+    opts = qcs_amd.FUSED_FP32
+    if args.fuse_k:
+        opts |= qcs_amd.FUSE_K(args.fuse_k)
+    if mode == "tiles":
+        opts |= qcs_amd.FUSE_TILES
+    if mode == "split":
+        opts |= qcs_amd.FUSE_SPLIT_BF16
+    tiled = mode in ("tiles", "split")
+    fused = qcs_amd.Circuit(n, opts)
+    hip.qcs_hip_set_tuning(fused.device_state, qcs_amd.TUNE_TILE_THREADS, args.tile_threads)
+    fuse_k = args.fuse_k or (4 if tiled else 5)
+    autotuned = tune and not args.fuse_k and not tiled
+    if autotuned:
+        # kernel variant per (n, k) and fusion width for this circuit
+        fused.apply(ops)
+        fuse_k = fused.autotune()
+        fused.flush()
+
+    def step():
+        fused.set_basis_state(0)
+        fused.apply(ops)
+        fused.flush()
+
+    for _ in range(warmup):
+        step()
+    state = fused.device_state
+    hip.qcs_hip_device_sync()
+    hip.qcs_hip_timing_enable(state, 1)
+    b0 = fused.get_num_fused_blocks()
+    p0 = fused.get_num_fused_passes()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    hip.qcs_hip_device_sync()
+    T = (time.perf_counter() - t0) / steps
+    f_ms, f_n, _, f_flops = timing(hip, state, qcs_amd.K_FUSED)
+    c_ms, _, _, _ = timing(hip, state, qcs_amd.K_COMPOSE)
+    hip.qcs_hip_timing_enable(state, 0)
+    avg_ms = f_ms / max(f_n, 1)
+    flops_per_launch = f_flops / max(f_n, 1)
+    r = {
+        "mode": mode,
+        "value": G * N / T,
+        "ms_per_step": T * 1e3,
+        "gates_per_s": G / T,
+        "fusion_k": fuse_k,
+        "autotuned": autotuned,
+        "tile_threads": args.tile_threads,
+        "fused_blocks": (fused.get_num_fused_blocks() - b0) // steps,
+        "hbm_passes": (fused.get_num_fused_passes() - p0) // steps,
+        "fused_kernel_ms_per_step": f_ms / steps,
+        "compose_ms_per_step": c_ms / steps,
+        "launches": f_n,
+        "avg_launch_ms": avg_ms,
+        "hbm_gbs": 2.0 * N * 8.0 / (avg_ms * 1e-3) / 1e9,
+        "tflops": flops_per_launch / (avg_ms * 1e-3) / 1e12,
+    }
+    return fused, r
+
+
+MODE_TEXT = {
+    "blocks": ("on FP32 MFMA, one HBM pass each", "k_fused_mfma (complex64 fused k-qubit pass)"),
+    "tiles": ("on FP32 MFMA, several per HBM pass (tile programs)",
+              "k_tile_program (complex64, fused blocks per tile pass, FP32 MFMA)"),
+    "split": ("on bf16 MFMA as split products (matrix 3 parts, state 2), several per HBM "
+              "pass (tile programs)",
+              "k_tile_program (complex64, fused blocks per tile pass, split-bf16 MFMA)"),
+}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -339,59 +420,19 @@ def main():
     ops = C.layered(n, depth, args.seed)
     G, N = len(ops), float(1 << n)
 
-    opts = qcs_amd.FUSED_FP32
-    if args.fuse_k:
-        opts |= qcs_amd.FUSE_K(args.fuse_k)
-    if args.mode == "tiles":
-        opts |= qcs_amd.FUSE_TILES
-    if args.mode == "split":
-        opts |= qcs_amd.FUSE_SPLIT_BF16
+    fused, m = run_mode(hip, ops, n, args.mode, args, args.steps, args.warmup,
+                        not args.no_tune)
     tiled = args.mode in ("tiles", "split")
-    fused = qcs_amd.Circuit(n, opts)
-    hip.qcs_hip_set_tuning(fused.device_state, qcs_amd.TUNE_TILE_THREADS, args.tile_threads)
-    fuse_k = args.fuse_k or (4 if tiled else 5)
-    if not args.no_tune and not args.fuse_k:
-        # kernel variant per (n, k) and fusion width for this circuit
-        fused.apply(ops)
-        fuse_k = fused.autotune()
-        fused.flush()
-
-    def step():
-        fused.set_basis_state(0)
-        fused.apply(ops)
-        fused.flush()
-
-    for _ in range(args.warmup):
-        step()
-    state = fused.device_state
-    hip.qcs_hip_device_sync()
-    hip.qcs_hip_timing_enable(state, 1)
-    b0 = fused.get_num_fused_blocks()
-    p0 = fused.get_num_fused_passes()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    hip.qcs_hip_device_sync()
-    T = (time.perf_counter() - t0) / args.steps
-    nblocks = (fused.get_num_fused_blocks() - b0) // args.steps
-    npasses = (fused.get_num_fused_passes() - p0) // args.steps
-    f_ms, f_n, f_bytes, f_flops = timing(hip, state, qcs_amd.K_FUSED)
-    c_ms, c_n, _, _ = timing(hip, state, qcs_amd.K_COMPOSE)
-    hip.qcs_hip_timing_enable(state, 0)
-    avg_ms = f_ms / max(f_n, 1)
-    bytes_per_launch = 2.0 * N * 8.0
-    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    flops_per_launch = f_flops / max(f_n, 1)
-    tflops = flops_per_launch / (avg_ms * 1e-3) / 1e12
-
+    print(f"bench: {args.mode} {m['ms_per_step']:.1f} ms per step", file=sys.stderr,
+          flush=True)
     result = {
         "metric": METRIC,
-        "value": G * N / T,
+        "value": m["value"],
         "unit": "amplitude updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": T * 1e3,
+        "ms_per_step": m["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -402,41 +443,41 @@ def main():
             "workload": (f"{n}-qubit layered Rz-Ry-Rz + CNOT-ladder circuit, depth "
                          f"{depth}, angles splitmix64(seed {args.seed}), complex64 "
                          f"state ({(8 << n) / 2**30:.0f} GiB), gates fused into "
-                         f"k<={fuse_k} unitaries on "
-                         + ("bf16 MFMA as split products (matrix 3 parts, state 2), "
-                            if args.mode == "split" else "FP32 MFMA, ")
-                         + ("several per HBM pass (tile programs)" if tiled
-                            else "one HBM pass each")),
+                         f"k<={m['fusion_k']} unitaries " + MODE_TEXT[args.mode][0]),
             "qubits": n,
             "depth": depth,
             "gates": G,
-            "fusion_k": fuse_k,
-            "autotuned": not args.no_tune and not args.fuse_k,
+            "fusion_k": m["fusion_k"],
+            "autotuned": m["autotuned"],
             "fusion_mode": args.mode,
             "tile_threads": args.tile_threads,
-            "fused_blocks": nblocks,
-            "hbm_passes": npasses,
+            "fused_blocks": m["fused_blocks"],
+            "hbm_passes": m["hbm_passes"],
         },
-        "gates_per_s": G / T,
-        "fused_kernel_ms_per_step": f_ms / args.steps,
-        "compose_ms_per_step": c_ms / args.steps,
+        "gates_per_s": m["gates_per_s"],
+        "fused_kernel_ms_per_step": m["fused_kernel_ms_per_step"],
+        "compose_ms_per_step": m["compose_ms_per_step"],
     }
-    result["roofline"] = {
-        "bound": "hbm",
-        "kernel": ("k_tile_program (complex64, fused blocks per tile pass)"
-                   if tiled else "k_fused_mfma (complex64 fused k-qubit pass)"),
-        "achieved": achieved,
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": achieved / HBM_PEAK_GBS,
-        "traffic": None,
-        "launches": f_n,
-        "avg_launch_ms": avg_ms,
-        "mfma_tflops": tflops,
-        # split mode: four bf16 products per complex-GEMM flop
-        "mfma_frac": (4.0 * tflops / BF16_MFMA_PEAK_TFLOPS if args.mode == "split"
-                      else tflops / FP32_MFMA_PEAK_TFLOPS),
-    }
+    # the dominant kernel's roofline: one-block passes stream the state once
+    # per block (HBM-bound); tile programs apply several blocks per pass and
+    # are bound by the matrix cores
+    if tiled:
+        peak = BF16_MFMA_PEAK_TFLOPS / 4.0 if args.mode == "split" else FP32_MFMA_PEAK_TFLOPS
+        result["roofline"] = {
+            "bound": "mfma", "kernel": MODE_TEXT[args.mode][1],
+            "achieved": m["tflops"], "peak": peak, "unit": "TFLOP/s",
+            "frac": m["tflops"] / peak, "traffic": None,
+            "launches": m["launches"], "avg_launch_ms": m["avg_launch_ms"],
+            "hbm_gbs": m["hbm_gbs"], "hbm_frac": m["hbm_gbs"] / HBM_PEAK_GBS,
+        }
+    else:
+        result["roofline"] = {
+            "bound": "hbm", "kernel": MODE_TEXT[args.mode][1],
+            "achieved": m["hbm_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": m["hbm_gbs"] / HBM_PEAK_GBS, "traffic": None,
+            "launches": m["launches"], "avg_launch_ms": m["avg_launch_ms"],
+            "mfma_tflops": m["tflops"], "mfma_frac": m["tflops"] / FP32_MFMA_PEAK_TFLOPS,
+        }
     traffic_file = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(traffic_file):
         with open(traffic_file) as fh:
@@ -444,6 +485,7 @@ def main():
         if t:
             result["roofline"]["traffic"] = t.get("bytes_per_launch")
 
+    plain = None
     if not args.no_plain:
         plain = qcs_amd.Circuit(n)
         pstate = plain.device_state
@@ -466,14 +508,38 @@ def main():
             "k_cu_gbs": (b2 / max(n2, 1)) / (m2 / max(n2, 1) * 1e-3) / 1e9 if n2 else None,
         }
         result["plain_fp64"]["k_1q_hbm_frac"] = (result["plain_fp64"]["k_1q_gbs"] or 0) / HBM_PEAK_GBS
-        result["speedup_vs_plain_fp64"] = Tp / T
+        result["speedup_vs_plain_fp64"] = Tp / m["ms_per_step"] * 1e3
         result["one_minus_f"] = omf
         result["max_abs_err"] = mx
+    fused.close()
+
+    # the other fused modes on the same circuit, for comparison (not `value`):
+    # one pass per block, and split-bf16 tile programs (bf16 matrix cores,
+    # the state rounded to 16 significant bits per block)
+    if not args.no_alt:
+        alt = {}
+        for mode in ("tiles", "blocks", "split"):
+            if mode == args.mode:
+                continue
+            f2, m2 = run_mode(hip, ops, n, mode, args, min(args.steps, 3), 1,
+                              not args.no_tune)
+            if plain is not None:
+                m2["one_minus_f"], m2["max_abs_err"] = f2.compare_with(plain)
+            f2.close()
+            alt[mode] = {k: m2[k] for k in ("value", "ms_per_step", "fused_blocks",
+                                            "hbm_passes", "avg_launch_ms", "hbm_gbs",
+                                            "tflops", "fusion_k")}
+            for k in ("one_minus_f", "max_abs_err"):
+                if k in m2:
+                    alt[mode][k] = m2[k]
+            print(f"bench: {mode} {m2['ms_per_step']:.1f} ms per step", file=sys.stderr,
+                  flush=True)
+        result["other_modes"] = alt
+    if plain is not None:
         plain.close()
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
-    fused.close()
     print(json.dumps(result), flush=True)
 
 
