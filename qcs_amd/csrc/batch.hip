@@ -1289,46 +1289,26 @@ int batch_run(qcs_hip_batch *b, const qcs_hip_batch_circuit *cs, int nc,
     build_part(0);
     for (auto &x : th) x.join();
   }
-  std::vector<BGate> hg;
-  std::vector<BGroup> hgrp;
-  std::vector<BBlock> hb;
-  {
-    size_t ng_all = 0, ngrp_all = 0, nb_all = 0;
-    for (const Part &pt : parts) {
-      ng_all += pt.g.size();
-      ngrp_all += pt.grp.size();
-      nb_all += pt.blk.size();
-    }
-    hg.reserve(ng_all);
-    hgrp.reserve(ngrp_all);
-    hb.reserve(nb_all);
+  const double tb1 = host_ms();
+  // each part's ops go to a contiguous range of the joined arrays: offsets
+  std::vector<size_t> og(size_t(nparts) + 1, 0), ogrp(size_t(nparts) + 1, 0),
+      ob(size_t(nparts) + 1, 0);
+  for (int t = 0; t < nparts; ++t) {
+    og[size_t(t) + 1] = og[size_t(t)] + parts[size_t(t)].g.size();
+    ogrp[size_t(t) + 1] = ogrp[size_t(t)] + parts[size_t(t)].grp.size();
+    ob[size_t(t) + 1] = ob[size_t(t)] + parts[size_t(t)].blk.size();
   }
-  for (Part &pt : parts) {
-    const int og = int(hg.size()), ogrp = int(hgrp.size()), ob = int(hb.size());
-    for (int p : pt.own) {
-      const int i = owner[size_t(p)];
-      if (cs[i].precision == QCS_HIP_C128 && cs[i].tile_bits >= kGR)
-        hp[size_t(p)].first_op += ogrp;
-      else if (cs[i].precision == QCS_HIP_C128)
-        hp[size_t(p)].first_op += og;
-      else
-        hp[size_t(p)].first_op += ob;
-    }
-    for (BGroup &grp : pt.grp) grp.first_gate += og;
-    hg.insert(hg.end(), pt.g.begin(), pt.g.end());
-    hgrp.insert(hgrp.end(), pt.grp.begin(), pt.grp.end());
-    hb.insert(hb.end(), pt.blk.begin(), pt.blk.end());
-    Part().g.swap(pt.g);
-  }
+  const size_t n_gate = og[size_t(nparts)], n_group = ogrp[size_t(nparts)],
+               n_block = ob[size_t(nparts)];
 
+  const double tb2 = host_ms();
   // composition inputs: the blocks and only the gates they use (complex128
-  // circuits never reach the composer)
+  // circuits never reach the composer), gates renumbered block by block
   std::vector<qcs_hip_block> cblk(blocks, blocks + nb);
-  std::vector<qcs_hip_gate> cgate;
+  size_t n_cgate = 0;
   for (int bi = 0; bi < nb; ++bi) {
-    cblk[size_t(bi)].first_gate = int(cgate.size());
-    cgate.insert(cgate.end(), gates + blocks[bi].first_gate,
-                 gates + blocks[bi].first_gate + blocks[bi].num_gates);
+    cblk[size_t(bi)].first_gate = int(n_cgate);
+    n_cgate += size_t(blocks[bi].num_gates);
   }
 
   // layout of the descriptor block (device and pinned staging alike)
@@ -1342,10 +1322,10 @@ int batch_run(qcs_hip_batch *b, const qcs_hip_batch_circuit *cs, int nc,
   L.circ = take(sizeof(BCirc) * size_t(nc));
   L.order = take(sizeof(int) * size_t(nc));
   L.pass = take(sizeof(BPass) * size_t(np));
-  L.gate = take(sizeof(BGate) * hg.size());
-  L.group = take(sizeof(BGroup) * hgrp.size());
-  L.block = take(sizeof(BBlock) * hb.size());
-  L.hgate = take(sizeof(qcs_hip_gate) * cgate.size());
+  L.gate = take(sizeof(BGate) * n_gate);
+  L.group = take(sizeof(BGroup) * n_group);
+  L.block = take(sizeof(BBlock) * n_block);
+  L.hgate = take(sizeof(qcs_hip_gate) * n_cgate);
   L.hblock = take(sizeof(qcs_hip_block) * size_t(nb));
   L.u = take(sizeof(double) * size_t(nc) * size_t(shots));
   L.total = off;
@@ -1383,6 +1363,7 @@ int batch_run(qcs_hip_batch *b, const qcs_hip_batch_circuit *cs, int nc,
       1, (uint64_t(1) << max_n) / std::min<uint64_t>(uint64_t(1) << max_n, kChunk)));
   const int lds_bytes = max_tile_bytes + int(2 * sizeof(double)) * nch_cap;
 
+  const double tb3 = host_ms();
   // buffers
   if ((rc = grow_device(b, &b->d_desc, &b->desc_cap, L.total))) return rc;
   if ((rc = grow_pinned(b, reinterpret_cast<void **>(&b->h_stage), &b->stage_cap,
@@ -1407,6 +1388,7 @@ int batch_run(qcs_hip_batch *b, const qcs_hip_batch_circuit *cs, int nc,
     b->h_out = static_cast<long *>(p);
   }
 
+  const double tb4 = host_ms();
   // descriptors into the pinned staging block
   char *h = b->h_stage;
   BCirc *hc = reinterpret_cast<BCirc *>(h + L.circ);
@@ -1423,13 +1405,45 @@ int batch_run(qcs_hip_batch *b, const qcs_hip_batch_circuit *cs, int nc,
     d.state_off = slot[i];
     d.shot_off = uint64_t(i) * uint64_t(shots);
   }
+  // every part writes its ops straight into the staging block (and fixes up
+  // the op indices of the passes it owns), and a share of the composer's
+  // gates; parts own disjoint passes and ranges
+  auto fill_part = [&](int t) {
+    Part &pt = parts[size_t(t)];
+    const int o_g = int(og[size_t(t)]), o_grp = int(ogrp[size_t(t)]), o_b = int(ob[size_t(t)]);
+    for (int p : pt.own) {
+      const int i = owner[size_t(p)];
+      if (cs[i].precision == QCS_HIP_C128 && cs[i].tile_bits >= kGR)
+        hp[size_t(p)].first_op += o_grp;
+      else if (cs[i].precision == QCS_HIP_C128)
+        hp[size_t(p)].first_op += o_g;
+      else
+        hp[size_t(p)].first_op += o_b;
+    }
+    for (BGroup &grp : pt.grp) grp.first_gate += o_g;
+    if (!pt.g.empty())
+      std::memcpy(h + L.gate + sizeof(BGate) * size_t(o_g), pt.g.data(),
+                  sizeof(BGate) * pt.g.size());
+    if (!pt.grp.empty())
+      std::memcpy(h + L.group + sizeof(BGroup) * size_t(o_grp), pt.grp.data(),
+                  sizeof(BGroup) * pt.grp.size());
+    if (!pt.blk.empty())
+      std::memcpy(h + L.block + sizeof(BBlock) * size_t(o_b), pt.blk.data(),
+                  sizeof(BBlock) * pt.blk.size());
+    const int b0 = int(int64_t(nb) * t / nparts), b1 = int(int64_t(nb) * (t + 1) / nparts);
+    qcs_hip_gate *cg = reinterpret_cast<qcs_hip_gate *>(h + L.hgate);
+    for (int bi = b0; bi < b1; ++bi)
+      if (blocks[bi].num_gates > 0)
+        std::memcpy(cg + cblk[size_t(bi)].first_gate, gates + blocks[bi].first_gate,
+                    sizeof(qcs_hip_gate) * size_t(blocks[bi].num_gates));
+  };
+  {
+    std::vector<std::thread> th;
+    for (int t = 1; t < nparts; ++t) th.emplace_back(fill_part, t);
+    fill_part(0);
+    for (auto &x : th) x.join();
+  }
   if (np > 0) std::memcpy(h + L.pass, hp.data(), sizeof(BPass) * hp.size());
-  if (!hg.empty()) std::memcpy(h + L.gate, hg.data(), sizeof(BGate) * hg.size());
-  if (!hgrp.empty())
-    std::memcpy(h + L.group, hgrp.data(), sizeof(BGroup) * hgrp.size());
-  if (!hb.empty()) std::memcpy(h + L.block, hb.data(), sizeof(BBlock) * hb.size());
-  if (!cgate.empty())
-    std::memcpy(h + L.hgate, cgate.data(), sizeof(qcs_hip_gate) * cgate.size());
   if (nb > 0) std::memcpy(h + L.hblock, cblk.data(), sizeof(qcs_hip_block) * size_t(nb));
   if (shots > 0)
     std::memcpy(h + L.u, u, sizeof(double) * size_t(nc) * size_t(shots));
@@ -1448,8 +1462,8 @@ int batch_run(qcs_hip_batch *b, const qcs_hip_batch_circuit *cs, int nc,
   if (per_cu < 1) return fail(QCS_HIP_EDEVICE, "batch kernel does not fit on a CU");
   const int grid = std::min(nc, per_cu * b->num_cus);
 
-  std::vector<long> key = {b->generation, nc, np, nb, long(cgate.size()), long(hg.size()),
-                           long(hgrp.size()), long(hb.size()), shots, b->nth,
+  std::vector<long> key = {b->generation, nc, np, nb, long(n_cgate), long(n_gate),
+                           long(n_group), long(n_block), shots, b->nth,
                            max_tile_bytes, nch_cap, grid, long(L.total),
                            long(states_bytes)};
   const bool new_graph = (b->flags & QCS_HIP_BATCH_GRAPH) && b->graph_key != key;
@@ -1468,9 +1482,11 @@ int batch_run(qcs_hip_batch *b, const qcs_hip_batch_circuit *cs, int nc,
   const char *prof = std::getenv("QCS_BATCH_PROFILE");
   if (prof && prof[0] && prof[0] != '0')
     std::fprintf(stderr,
-                 "qcs_hip_batch_run: validate %.1f ms, descriptors %.1f ms, "
-                 "device %.1f ms (kernel %.1f), wall to sync %.1f ms\n",
-                 t1 - t0, t2 - t1, b->total_ms, b->kernel_ms, t3 - t2);
+                 "qcs_hip_batch_run: validate %.1f ms, descriptors %.1f ms (parts %.1f, "
+                 "offsets %.1f, compose inputs + layout %.1f, buffers %.1f, staging %.1f), "
+                 "device %.1f ms (kernel %.1f), wall to sync %.1f ms, %.1f MB staged\n",
+                 t1 - t0, t2 - t1, tb1 - t1, tb2 - tb1, tb3 - tb2, tb4 - tb3, t2 - tb4,
+                 b->total_ms, b->kernel_ms, t3 - t2, double(L.total) / 1e6);
   b->last_n.assign(size_t(nc), 0);
   b->last_prec.assign(size_t(nc), 0);
   b->last_off = slot;

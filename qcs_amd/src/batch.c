@@ -262,6 +262,74 @@ static int host_threads(int count) {
   return (int)t;
 }
 
+/* the concatenated plan: circuit i's passes / blocks / gates start at
+   pass_off[i] / block_off[i] / gate_off[i] */
+typedef struct {
+  t_q_circuit **circuits;
+  const circuit_plan *plans;
+  const long *pass_off, *block_off, *gate_off;
+  qcs_hip_batch_circuit *circ;
+  qcs_hip_pass *pass;
+  qcs_hip_block *block;
+  qcs_hip_gate *gate;
+  int count, parts;
+} concat_job;
+
+typedef struct {
+  concat_job *job;
+  int part;
+} concat_arg;
+
+static void *concat_worker(void *arg) {
+  const concat_arg *a = (const concat_arg *)arg;
+  const concat_job *job = a->job;
+  const int i0 = (int)((long)job->count * a->part / job->parts);
+  const int i1 = (int)((long)job->count * (a->part + 1) / job->parts);
+  int i, j;
+  for (i = i0; i < i1; i++) {
+    const circuit_plan *cp = &job->plans[i];
+    const t_q_circuit *c = job->circuits[i];
+    const int fused = (c->options & QCS_FUSED_FP32) != 0;
+    const long np = job->pass_off[i], nb = job->block_off[i], ng = job->gate_off[i];
+    job->circ[i].num_qubits = c->num_qubits;
+    job->circ[i].precision = fused ? QCS_HIP_C64 : QCS_HIP_C128;
+    job->circ[i].tile_bits = cp->tb;
+    job->circ[i].first_pass = (int)np;
+    job->circ[i].num_passes = cp->np;
+    for (j = 0; j < cp->np; j++) {
+      job->pass[np + j] = cp->passes[j];
+      job->pass[np + j].first_block += (int)(fused ? nb : ng);
+    }
+    for (j = 0; j < cp->nb; j++) {
+      job->block[nb + j] = cp->blocks[j];
+      job->block[nb + j].first_gate += (int)ng;
+    }
+    if (c->pending_size > 0)
+      memcpy(&job->gate[ng], cp->ordered ? cp->ordered : c->pending,
+             (size_t)c->pending_size * sizeof(qcs_hip_gate));
+  }
+  return NULL;
+}
+
+static void concat_all(concat_job *job) {
+  pthread_t th[QCS_BATCH_MAX_THREADS];
+  concat_arg args[QCS_BATCH_MAX_THREADS];
+  int started[QCS_BATCH_MAX_THREADS];
+  int nt = host_threads(job->count), t;
+  job->parts = nt;
+  for (t = 0; t < nt; t++) {
+    args[t].job = job;
+    args[t].part = t;
+  }
+  for (t = 1; t < nt; t++)
+    started[t] = pthread_create(&th[t], NULL, concat_worker, &args[t]) == 0;
+  concat_worker(&args[0]);
+  for (t = 1; t < nt; t++) {
+    if (started[t]) pthread_join(th[t], NULL);
+    else concat_worker(&args[t]); /* no thread: do that part here */
+  }
+}
+
 static void plan_all(const t_q_batch *b, t_q_circuit **circuits, int count,
                      circuit_plan *plans) {
   plan_job job;
@@ -287,9 +355,11 @@ int qc_batch_sample(t_q_batch *batch, t_q_circuit **circuits, int count,
   qcs_hip_pass *pass = NULL;
   qcs_hip_block *block = NULL;
   qcs_hip_gate *gate = NULL;
+  long *offs = NULL;
+  concat_job job;
   double *u = NULL;
   long total, s, np = 0, nb = 0, ng = 0;
-  int i, j, rc = QCS_HIP_ENOMEM;
+  int i, rc = QCS_HIP_ENOMEM;
   double t0, t1, t2, t3;
   if (!batch || count < 0 || shots < 0 || (count > 0 && !circuits) ||
       (count > 0 && shots > 0 && !indices))
@@ -323,8 +393,9 @@ int qc_batch_sample(t_q_batch *batch, t_q_circuit **circuits, int count,
   batch->last_valid = 0;
   plans = (circuit_plan *)calloc((size_t)count, sizeof(circuit_plan));
   circ = (qcs_hip_batch_circuit *)calloc((size_t)count, sizeof(qcs_hip_batch_circuit));
+  offs = (long *)malloc(3 * (size_t)count * sizeof(long));
   if (total > 0) u = (double *)malloc((size_t)total * sizeof(double));
-  if (!plans || !circ || (total > 0 && !u)) goto out;
+  if (!plans || !circ || !offs || (total > 0 && !u)) goto out;
   t0 = now_ms();
   plan_all(batch, circuits, count, plans);
   t1 = now_ms();
@@ -333,6 +404,9 @@ int qc_batch_sample(t_q_batch *batch, t_q_circuit **circuits, int count,
       fprintf(stderr, "Error: planning batch circuit %d failed.\n", i);
       goto out;
     }
+    offs[i] = np;
+    offs[count + i] = nb;
+    offs[2 * count + i] = ng;
     np += plans[i].np;
     nb += plans[i].nb;
     ng += circuits[i]->pending_size;
@@ -345,33 +419,18 @@ int qc_batch_sample(t_q_batch *batch, t_q_circuit **circuits, int count,
   block = (qcs_hip_block *)malloc((size_t)(nb > 0 ? nb : 1) * sizeof(qcs_hip_block));
   gate = (qcs_hip_gate *)malloc((size_t)(ng > 0 ? ng : 1) * sizeof(qcs_hip_gate));
   if (!pass || !block || !gate) goto out;
-  /* concatenate: pass / block / gate offsets of circuit i */
-  np = nb = ng = 0;
-  for (i = 0; i < count; i++) {
-    const circuit_plan *cp = &plans[i];
-    const t_q_circuit *c = circuits[i];
-    const int fused = (c->options & QCS_FUSED_FP32) != 0;
-    const int cng = c->pending_size;
-    circ[i].num_qubits = c->num_qubits;
-    circ[i].precision = fused ? QCS_HIP_C64 : QCS_HIP_C128;
-    circ[i].tile_bits = cp->tb;
-    circ[i].first_pass = (int)np;
-    circ[i].num_passes = cp->np;
-    for (j = 0; j < cp->np; j++) {
-      pass[np + j] = cp->passes[j];
-      pass[np + j].first_block += (int)(fused ? nb : ng);
-    }
-    for (j = 0; j < cp->nb; j++) {
-      block[nb + j] = cp->blocks[j];
-      block[nb + j].first_gate += (int)ng;
-    }
-    if (cng > 0)
-      memcpy(&gate[ng], cp->ordered ? cp->ordered : c->pending,
-             (size_t)cng * sizeof(qcs_hip_gate));
-    np += cp->np;
-    nb += cp->nb;
-    ng += cng;
-  }
+  /* concatenate on host threads: circuit i's ranges start at its offsets */
+  job.circuits = circuits;
+  job.plans = plans;
+  job.pass_off = offs;
+  job.block_off = offs + count;
+  job.gate_off = offs + 2 * (size_t)count;
+  job.circ = circ;
+  job.pass = pass;
+  job.block = block;
+  job.gate = gate;
+  job.count = count;
+  concat_all(&job);
   t2 = now_ms();
   for (s = 0; s < total; s++) u[s] = rand() / (double)RAND_MAX;
   t3 = now_ms();
@@ -389,6 +448,7 @@ out:
   if (plans)
     for (i = 0; i < count; i++) free_plan(&plans[i]);
   free(plans);
+  free(offs);
   free(circ);
   free(pass);
   free(block);
