@@ -55,6 +55,7 @@
 // Work-group tile order is XCD aware: the hardware places work-group i on
 // XCD i % 8, so XCD x is given the contiguous x-th eighth of the tiles.
 #include <cstring>
+#include <mutex>
 
 #include "device.h"
 #include "internal.h"
@@ -484,20 +485,29 @@ static int grow_dev(void **p, size_t *cap, size_t need) {
 template <int NTH, bool SPLIT>
 static int launch_tp(qcs_hip_state *s, const TilePass *dpass, int pi,
                      const TileBlock *dblocks, uint64_t ntiles, int nblocks) {
-  static bool attr_set = false;
-  if (!attr_set) {
+  // once per kernel instance (states may launch from separate threads)
+  static std::once_flag once;
+  static int attr_rc = 0;
+  std::call_once(once, [] {
     const void *fn = reinterpret_cast<const void *>(&k_tile_program<kTileBitsFixed, NTH, SPLIT>);
-    QCS_HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    int(kLdsPerCu)));
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       int(kLdsPerCu));
+    if (e != hipSuccess) {
+      attr_rc = check_hip(e, "hipFuncSetAttribute");
+      return;
+    }
     // LDS is addressed by absolute byte address: the dynamic LDS must start
     // at 0, i.e. the kernel has no static LDS
     hipFuncAttributes fa;
-    QCS_HIP_TRY(hipFuncGetAttributes(&fa, fn));
+    if ((e = hipFuncGetAttributes(&fa, fn)) != hipSuccess) {
+      attr_rc = check_hip(e, "hipFuncGetAttributes");
+      return;
+    }
     if (fa.sharedSizeBytes != 0)
-      return fail(QCS_HIP_EDEVICE, "k_tile_program has %zu bytes of static LDS",
-                  size_t(fa.sharedSizeBytes));
-    attr_set = true;
-  }
+      attr_rc = fail(QCS_HIP_EDEVICE, "k_tile_program has %zu bytes of static LDS",
+                     size_t(fa.sharedSizeBytes));
+  });
+  if (attr_rc) return attr_rc;
   const size_t lds =
       kTileLdsTile + size_t(SPLIT ? kTileStrideSplit : kTileStrideFp32) * size_t(nblocks);
   const uint64_t per_cu = kLdsPerCu / lds;  // work-groups that fit per CU

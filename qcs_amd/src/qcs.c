@@ -125,8 +125,10 @@ static void add_history(t_q_circuit *c, const char *name, int target,
 
 static long g_version_stamp = 0;
 
+/* a stamp no other change has had: the counter is shared by every circuit,
+   which separate threads may record into at once */
 void qcs_touch(t_q_circuit *c) {
-  if (c) c->version = ++g_version_stamp;
+  if (c) c->version = __atomic_add_fetch(&g_version_stamp, 1, __ATOMIC_RELAXED);
 }
 
 static void record(t_q_circuit *c, int kind, int target, int control,
