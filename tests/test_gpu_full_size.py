@@ -13,7 +13,11 @@ a CPU run of the whole state:
     an FP64 norm + renormalisation at every layer (QCS_RENORM_AT_BARRIER),
     then its inverse; 1 - |<0|U^-1 U|0>|^2 bounds the accumulated FP32 error
     of both halves (a complex128 run at 34 qubits needs 256 GiB and does not
-    fit beside it)."""
+    fit beside it).
+  * 30 and 34 qubits, complex64 tile programs (FP32 and split-bf16): the
+    depth-20 layered circuit and its inverse in one flush each; the split
+    form rounds the state to 16 significant bits per block, so its bound is
+    the form's 1e-6 rather than the FP32 forms' 1e-9."""
 import math
 
 import numpy as np
@@ -139,3 +143,26 @@ def test_34q_fused_with_layer_renorm_round_trip(qlib):
         print(f"[full] fused n={n}: 1 - |<0|U^-1 U|0>|^2 = {1 - p0:.3e}, "
               f"blocks {c.get_num_fused_blocks()}", flush=True)
         assert 1.0 - p0 <= 2e-6
+
+
+@pytest.mark.parametrize("n,mode", [(30, "tiles"), (30, "split"), (34, "split")])
+def test_tile_programs_round_trip(qlib, n, mode):
+    print(f"[full] tile programs round trip n={n} mode={mode}", flush=True)
+    ops = C.layered(n, 20, 2)
+    opt = qcs_amd.FUSE_TILES if mode == "tiles" else qcs_amd.FUSE_SPLIT_BF16
+    with qcs_amd.Circuit(n, qcs_amd.FUSED_FP32 | opt) as c:
+        c.apply(ops)
+        c.flush()
+        assert abs(c.norm_sq() - 1.0) < 1e-6
+        c.apply(inverse(ops))
+        a0 = c.amplitudes(0, 1)[0]
+        # the weight outside |0...0>, summed in FP64 with a0 taken out: 1 - F
+        # without the cancellation of 1 - |a0|^2 at FP32 resolution
+        c.set_amplitudes(np.zeros(1, np.complex128), 0)
+        rest = c.norm_sq()
+        omf = rest / (rest + abs(a0) ** 2)
+        print(f"[full] {mode} n={n}: 1 - |<0|U^-1 U|0>|^2 = {omf:.3e}, "
+              f"blocks {c.get_num_fused_blocks()}, passes {c.get_num_fused_passes()}",
+              flush=True)
+        assert c.get_num_fused_passes() < c.get_num_fused_blocks()
+        assert omf <= (1e-6 if mode == "split" else 1e-9)
