@@ -14,22 +14,30 @@
 // and imaginary parts at PLANE = 32 KiB; tile element e (bit j of e = index
 // bit S[j]) sits at slot swz(e) = e ^ (e>>5 & 31) ^ (e>>10 & 7) of both.
 // swz is linear over GF(2), so the byte address of (column c, row s) of a
-// block is 4 swz(c) ^ 4 swz(row bits of s): the host precomputes the row
-// terms of every block, and since the kernel has no static LDS (checked on
-// the host) its dynamic LDS starts at address 0, so each B-operand read or
-// result write costs one XOR and no base add.  The fold puts index bits
-// 5..12 onto the low slot bits, so the 32 columns a half-wave reads (the 5
-// lowest non-target tile bits) are 32 distinct slots mod 32; in separate
-// planes those are 32 distinct 4-byte banks.
+// block is 4 swz(c) ^ 4 swz(row bits of s), and since the kernel has no
+// static LDS (checked on the host) its dynamic LDS starts at address 0:
+// each B-operand read or result write costs one XOR and no base add.
+//
+// Work split.  A block's column bits (the tile bits that are not targets)
+// are split on the host into 5 lane bits, the wave bits (the wave index) and
+// group bits (the wave's column groups), each part a table of swz terms
+// (TileBlock).  Each wave then touches only the amplitudes whose wave bits
+// hold its index, so consecutive blocks given the same wave bits need no
+// barrier between them: the host takes a pass's blocks in runs whose targets
+// leave enough tile bits free and gives each run one choice of wave bits,
+// the one with the fewest LDS bank conflicts over its blocks (the 32 columns
+// a half-wave reads fall in distinct banks when the lane bits land on
+// distinct low slot bits; a shared choice can leave a block 2-way
+// conflicted, which costs less than a barrier).
 //
 // After the tile come, per block of the pass, its A operands in lane order
-// and a 256-byte table: the per-lane part of the column address (column l32
-// with zero bits at the targets, through swz) and the byte offsets of the
-// lane's output rows.  Column c of a wave's group splits into that lane part
-// and a wave-uniform part computed on the scalar unit.  The operands stay
-// in LDS because on gfx9 every vector memory load retires through one
-// in-order counter, so a global load inside the tile loop would wait behind
-// the next tile's prefetch.
+// and the lane table (the lane part of the column address).  The FP32 form
+// reads a block's operands and lane part, and its scalars (row offsets of
+// rows 1, 2, 4, 8 -- the rest follow by linearity -- the wave part,
+// the width and the barrier flag), while the previous block's matrix-core
+// chain runs.  The operands stay in LDS because on gfx9 every vector memory
+// load retires through one in-order counter, so a global load inside the
+// tile loop would wait behind the next tile's prefetch.
 //
 // Vector ALU work does not run alongside FP32 MFMA on gfx950 (their times
 // add; bf16 MFMA overlaps it only in part), so the block code is written for
@@ -56,6 +64,7 @@
 // XCD i % 8, so XCD x is given the contiguous x-th eighth of the tiles.
 #include <cstring>
 #include <mutex>
+#include <vector>
 
 #include "device.h"
 #include "internal.h"
@@ -81,7 +90,7 @@ constexpr size_t kLdsPerCu = 160 * 1024;
 // per block: A operands, then the lane table
 constexpr int kTileAFp32 = 4 * 64 * 16;             // [4][lane] float4
 constexpr int kTileASplit = 3 * 2 * 64 * 16;        // [part][instr][lane] 8 x bf16
-constexpr int kTileAux = 256;                        // lanecol[32], outoff[2][8]
+constexpr int kTileAux = 128;                        // lanecol[32]
 constexpr int kTileStrideFp32 = kTileAFp32 + kTileAux;
 constexpr int kTileStrideSplit = kTileASplit + kTileAux;
 static_assert(kTileLdsTile + size_t(kTileStrideFp32) * QCS_HIP_TILE_MAX_BLOCKS <= kLdsPerCu,
@@ -98,13 +107,22 @@ struct TilePass {
   int sbits[kPassMaxBits];
 };
 
+// A block's columns (the tile bits that are not targets) split into three
+// sets of bit positions, chosen on the host: 5 lane bits (bit i of the
+// lane's l32), the wave bits (bit i of the wave index) and the group bits
+// (bit i of the wave's column-group counter j).  By linearity of swz the
+// byte address of (row r, lane, wave, j) is
+//   rowoff[r] ^ lanecol[l32] ^ wpart[wave] ^ jpart[j]   (+ PLANE for imag).
 struct TileBlock {
   int k;
   int wslot;
-  int tpos[kTileKMax];       // tile-bit positions of the targets, ascending
+  int sync;                  // 1: barrier after this block (the next block's
+                             //    waves own other amplitudes)
+  int reserved;
   int rowoff[kTileRMax];     // 4 swz(row bits of s): byte offset of row s
-  int lanecol[32];           // 4 swz(l32 with zero bits at the targets)
-  int outoff[2][8];          // [h][2q + j]: rowoff of output row 4q + 2h + j
+  int wpart[16];             // 4 swz(wave index on the wave bits)
+  int jpart[16];             // 4 swz(j on the group bits)
+  int lanecol[32];           // 4 swz(l32 on the lane bits)
 };
 
 __host__ __device__ __forceinline__ int swz(int e) {
@@ -119,11 +137,6 @@ __device__ __forceinline__ float4 ld_nt(const float2 *p) {
 __device__ __forceinline__ void st_nt(float2 *p, float4 x) {
   const f4v v = {x.x, x.y, x.z, x.w};
   __builtin_nontemporal_store(v, reinterpret_cast<f4v *>(p));
-}
-
-// insert a zero bit at position b (32-bit)
-__host__ __device__ __forceinline__ int ins0(int x, int b) {
-  return ((x >> b) << (b + 1)) | (x & ((1 << b) - 1));
 }
 
 // LDS by absolute byte address (the dynamic LDS starts at 0)
@@ -149,39 +162,82 @@ __device__ __forceinline__ void split_pair(float x0, float x1, unsigned &hi,
   lo = __builtin_bit_cast(unsigned, __builtin_convertvector(r, bf16x2));
 }
 
-// What every block kernel needs of its block: uniform row offsets and target
-// positions (scalar registers), the lane's column part and output offsets.
+// The uniform part of a block (scalar registers), read one block ahead so
+// that the scalar-cache latency is hidden behind the current block.  Row
+// offsets are linear in the row bits (swz is), so the offsets of rows 1, 2,
+// 4 and 8 give all of them.
+struct BlockScalars {
+  int k, sync, up;  // up: the wave's part of the column address
+  int rb[kTileKMax];
+  __device__ __forceinline__ int ro(int s) const {
+    int o = 0;
+#pragma unroll
+    for (int q = 0; q < kTileKMax; ++q)
+      if ((s >> q) & 1) o ^= rb[q];
+    return o;
+  }
+};
+
+__device__ __forceinline__ BlockScalars load_scalars(const TileBlock *__restrict__ bl,
+                                                     int wave) {
+  // (readfirstlane: the compiler otherwise carries them in vector registers
+  // from one block to the next)
+  BlockScalars d;
+  d.k = __builtin_amdgcn_readfirstlane(bl->k);
+  d.sync = __builtin_amdgcn_readfirstlane(bl->sync);
+  d.up = __builtin_amdgcn_readfirstlane(bl->wpart[wave]);
+#pragma unroll
+  for (int q = 0; q < kTileKMax; ++q)
+    d.rb[q] = __builtin_amdgcn_readfirstlane(bl->rowoff[1 << q]);
+  return d;
+}
+
+// The per-lane values of an FP32 block kept in LDS -- its A operands and the
+// lane part of the column address -- read one block ahead, while the
+// current block's matrix-core chain runs.
+struct BlockRegs {
+  f4v a[4];
+  int lp;
+};
+
+__device__ __forceinline__ void load_regs(BlockRegs &r, int blk) {
+  const int lane = threadIdx.x & 63, l32 = lane & 31;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) r.a[v] = lds_load<f4v>(blk + (v * 64 + lane) * 16);
+  r.lp = lds_load<int>(blk + kTileAFp32 + 4 * l32);
+}
+
+// What every block kernel needs of its block: uniform row offsets and the
+// wave's column part (scalar registers), the lane's column part and output
+// offsets.
 template <int K>
 struct BlockGeom {
   static constexpr int R = 1 << K;
-  int tpos[K];
   int ro[R];
-  int lp, lph;  // lane part of the column address: real plane, B plane
-  int os[8];    // output row offsets of this lane
-  __device__ __forceinline__ BlockGeom(const TileBlock *__restrict__ bl, int aux) {
-    const int lane = threadIdx.x & 63, h = lane >> 5, l32 = lane & 31;
+  int upw;      // wave part of the column address
+  int lph;      // lane part of the column address in the lane's B plane
+  int lpo;      // lane part of the output address: lanes of the upper half
+                // hold rows 4q + 2 + j, i.e. rowoff[4q + j] ^ rowoff[2]
+  __device__ __forceinline__ BlockGeom(const BlockScalars &d, int lp) {
+    const int h = (threadIdx.x >> 5) & 1;
 #pragma unroll
-    for (int q = 0; q < K; ++q) tpos[q] = bl->tpos[q];
-#pragma unroll
-    for (int s = 0; s < R; ++s) ro[s] = bl->rowoff[s];
-    lp = lds_load<int>(aux + 4 * l32);
+    for (int s = 0; s < R; ++s) ro[s] = d.ro(s);
+    upw = d.up;
     lph = lp | (h ? kTilePlane : 0);
-    const i32x4 o0 = lds_load<i32x4>(aux + 128 + 32 * h);
-    const i32x4 o1 = lds_load<i32x4>(aux + 128 + 32 * h + 16);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      os[j] = o0[j];
-      os[4 + j] = o1[j];
-    }
+    lpo = lp ^ (h ? d.rb[1] : 0);
   }
-  // wave-uniform part of the address of column group `grp` (32 columns)
-  __device__ __forceinline__ int group_part(int grp) const {
-    int up = grp * 32;
-#pragma unroll
-    for (int q = 0; q < K; ++q) up = ins0(up, tpos[q]);
-    return 4 * swz(up);
+  // from the lane table at LDS address aux
+  static __device__ __forceinline__ BlockGeom from_lds(const BlockScalars &d, int aux) {
+    return BlockGeom(d, lds_load<int>(aux + 4 * (threadIdx.x & 31)));
   }
-  // result rows of a 32x32 accumulator tile (rows 4q + 2h + j of the block)
+  // wave-uniform part of the address of the wave's column group j
+  template <int GROUPS>
+  __device__ __forceinline__ int group_part(const TileBlock *__restrict__ bl, int j) const {
+    if (GROUPS == 1) return upw;
+    return upw ^ bl->jpart[j];
+  }
+  // result rows of a 32x32 accumulator tile (rows 4q + 2h + j of the block);
+  // cb = lpo ^ the wave-uniform column part
   __device__ __forceinline__ void store(int cb, const floatx16 &acc) const {
     const int h = (threadIdx.x >> 5) & 1;
 #pragma unroll
@@ -190,7 +246,7 @@ struct BlockGeom {
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
         if (4 * q + 2 * h + j < R) {
-          const int o = cb ^ os[2 * q + j];
+          const int o = cb ^ ro[4 * q + j];
           lds_store<float>(o, acc[4 * q + 2 * j]);
           lds_store<float>(o + kTilePlane, acc[4 * q + 2 * j + 1]);
         }
@@ -201,26 +257,27 @@ struct BlockGeom {
 
 // One block in FP32: lane (l32, h) supplies A[l32][2s + h] (output row
 // l32/2, part l32&1; input row s, part h) and B[2s + h][column l32] -- the
-// lane's column of plane h, row s -- for s = 0 .. R-1, one MFMA each.
+// lane's column of plane h, row s -- for s = 0 .. R-1, one MFMA each.  The
+// next block's scalars and per-lane values (nd, nr) are read once the first
+// column group's chain is issued.
 template <int K, int NTH, int TILE>
-__device__ __forceinline__ void tp_block(const TileBlock *__restrict__ bl, int blk) {
+__device__ __forceinline__ void tp_block(const TileBlock *__restrict__ bl,
+                                         const BlockScalars &d, const BlockRegs &r,
+                                         const TileBlock *__restrict__ nbl, int nblk,
+                                         int wave, BlockScalars &nd, BlockRegs &nr) {
   constexpr int R = 1 << K, NW = NTH / 64, GROUPS = (TILE >> K) / 32 / NW;
-  static_assert(GROUPS >= 1, "a wave takes whole column groups");
-  const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  static_assert(GROUPS >= 1 && GROUPS <= 16, "a wave takes 1..16 column groups");
   float wa[R];
 #pragma unroll
-  for (int v = 0; v < (R + 3) / 4; ++v) {
-    const f4v x = lds_load<f4v>(blk + (v * 64 + lane) * 16);
-    wa[4 * v] = x.x;
-    if (4 * v + 1 < R) wa[4 * v + 1] = x.y;
-    if (4 * v + 2 < R) wa[4 * v + 2] = x.z;
-    if (4 * v + 3 < R) wa[4 * v + 3] = x.w;
+  for (int s = 0; s < R; ++s) wa[s] = r.a[s >> 2][s & 3];
+  const BlockGeom<K> geo(d, r.lp);
+  if (GROUPS > 1) {
+    nd = load_scalars(nbl, wave);
+    load_regs(nr, nblk);
   }
-  const BlockGeom<K> geo(bl, blk + kTileAFp32);
 #pragma unroll 1
   for (int g = 0; g < GROUPS; ++g) {
-    const int up = geo.group_part(wave + g * NW);
+    const int up = geo.template group_part<GROUPS>(bl, g);
     const int cbh = geo.lph ^ up;
     float bv[R];
 #pragma unroll
@@ -233,8 +290,33 @@ __device__ __forceinline__ void tp_block(const TileBlock *__restrict__ bl, int b
 #pragma unroll
     for (int s = 0; s < R; ++s)
       acc = __builtin_amdgcn_mfma_f32_32x32x2f32(wa[s], bv[s], acc, 0, 0, 0);
-    geo.store(geo.lp ^ up, acc);
+    __builtin_amdgcn_sched_barrier(0);
+    if (GROUPS == 1) {
+      nd = load_scalars(nbl, wave);
+      load_regs(nr, nblk);
+    }
+    geo.store(geo.lpo ^ up, acc);
   }
+}
+
+// One FP32 block of the pass (index b), with the next block's values read
+// into (nd, nr); the block after the last is the pass's first (next tile).
+template <int NTH, int TILE>
+__device__ __forceinline__ void tp_step(const TileBlock *__restrict__ blocks, int b, int nb,
+                                        int ops, int wave, const BlockScalars &d,
+                                        const BlockRegs &r, BlockScalars &nd, BlockRegs &nr) {
+  const int bn = b + 1 < nb ? b + 1 : 0;
+  const TileBlock *bl = blocks + b, *nbl = blocks + bn;
+  const int nblk = ops + bn * kTileStrideFp32;
+  switch (d.k) {
+    case 1: tp_block<1, NTH, TILE>(bl, d, r, nbl, nblk, wave, nd, nr); break;
+    case 2: tp_block<2, NTH, TILE>(bl, d, r, nbl, nblk, wave, nd, nr); break;
+    case 3: tp_block<3, NTH, TILE>(bl, d, r, nbl, nblk, wave, nd, nr); break;
+    default: tp_block<4, NTH, TILE>(bl, d, r, nbl, nblk, wave, nd, nr); break;
+  }
+  // consecutive blocks whose waves own the same amplitudes (same wave bits)
+  // need no barrier between them: each wave's LDS accesses are in order
+  if (d.sync) __syncthreads();
 }
 
 // One block in split-bf16 form (see the file comment).  Lane (l32, h)
@@ -244,13 +326,14 @@ __device__ __forceinline__ void tp_block(const TileBlock *__restrict__ bl, int b
 // groups at once: G independent accumulation chains, so one chain's split
 // overlaps another's matrix-core work.
 template <int K, int NTH, int TILE>
-__device__ __forceinline__ void tp_block_split(const TileBlock *__restrict__ bl, int blk) {
+__device__ __forceinline__ void tp_block_split(const TileBlock *__restrict__ bl,
+                                               const BlockScalars &d, int blk) {
   constexpr int R = 1 << K, NC = R > 8 ? 2 : 1, NW = NTH / 64;
   constexpr int GROUPS = (TILE >> K) / 32 / NW;
   constexpr int G = GROUPS >= 2 ? 2 : 1;
-  static_assert(GROUPS >= 1 && GROUPS % G == 0, "column groups must split evenly");
+  static_assert(GROUPS >= 1 && GROUPS <= 16 && GROUPS % G == 0,
+                "column groups must split evenly");
   const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   bf16x8 wh[NC], wm[NC], wl[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
@@ -258,14 +341,14 @@ __device__ __forceinline__ void tp_block_split(const TileBlock *__restrict__ bl,
     wm[c] = lds_load<bf16x8>(blk + ((1 * 2 + c) * 64 + lane) * 16);
     wl[c] = lds_load<bf16x8>(blk + ((2 * 2 + c) * 64 + lane) * 16);
   }
-  const BlockGeom<K> geo(bl, blk + kTileASplit);
+  const BlockGeom<K> geo = BlockGeom<K>::from_lds(d, blk + kTileASplit);
 #pragma unroll 1
   for (int g0 = 0; g0 < GROUPS; g0 += G) {
     int up[G];
     float bv[G][8 * NC];
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-      up[g] = geo.group_part(wave + (g0 + g) * NW);
+      up[g] = geo.template group_part<GROUPS>(bl, g0 + g);
       const int cbh = geo.lph ^ up[g];
 #pragma unroll
       for (int s = 0; s < 8 * NC; ++s)
@@ -302,7 +385,7 @@ __device__ __forceinline__ void tp_block_split(const TileBlock *__restrict__ bl,
       }
     }
 #pragma unroll
-    for (int g = 0; g < G; ++g) geo.store(geo.lp ^ up[g], acc[g]);
+    for (int g = 0; g < G; ++g) geo.store(geo.lpo ^ up[g], acc[g]);
   }
 }
 
@@ -329,6 +412,7 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
   const TilePass &ps = passes[pass_index];
   const int nb = ps.nblocks;
   const int tid = threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 
   // stage the A operands and lane tables of every block
   for (int idx = tid; idx < nb * 64 * 16; idx += NTH) {
@@ -354,11 +438,9 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
       lds_store<__bf16>(blk + ((2 * 2 + c) * 64 + ln) * 16 + 2 * j, pl);
     }
   }
-  for (int idx = tid; idx < nb * 48; idx += NTH) {
-    const int b = idx / 48, e = idx % 48;
-    const TileBlock &bl = blocks[ps.first_block + b];
-    const int v = e < 32 ? bl.lanecol[e] : bl.outoff[(e - 32) >> 3][(e - 32) & 7];
-    lds_store<int>(OPS + b * STRIDE + ABYTES + 4 * e, v);
+  for (int idx = tid; idx < nb * 32; idx += NTH) {
+    const int b = idx / 32, e = idx % 32;
+    lds_store<int>(OPS + b * STRIDE + ABYTES + 4 * e, blocks[ps.first_block + b].lanecol[e]);
   }
 
   // element e = 2*tid + 2*NTH*it and its neighbour e+1 (tile bit 0)
@@ -371,11 +453,14 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
       e_t |= 1 << j;
     }
   const int slot_t = swz(e_t);
+  uint64_t ib[IB];  // index bit of each bit of `it`
+#pragma unroll
+  for (int jj = 0; jj < IB; ++jj) ib[jj] = uint64_t(1) << ps.sbits[TBITS + jj];
   auto it_addr = [&](int it) {
     uint64_t ai = 0;
 #pragma unroll
     for (int jj = 0; jj < IB; ++jj)
-      if ((it >> jj) & 1) ai |= uint64_t(1) << ps.sbits[TBITS + jj];
+      if ((it >> jj) & 1) ai |= ib[jj];
     return ai;
   };
 
@@ -383,12 +468,20 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
   const uint64_t xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
   uint64_t tile = xcd * T8 + slot;
   const uint64_t tend = (xcd + 1) * T8;
-  auto tile_base = [&](uint64_t t) {
+  // tile t's base address deposits t on the index bits outside the tile
+  // (cmask); the next tile's, t + G8, is found by adding the deposited G8 in
+  // the deposited space: ((x | ~cmask) + dinc) & cmask
+  auto deposit = [&](uint64_t t) {
     uint64_t b = t;
 #pragma unroll
     for (int j = 0; j < TB; ++j) b = insert0(b, ps.sbits[j]);
     return b;
   };
+  uint64_t tmask = 0;
+#pragma unroll
+  for (int j = 0; j < TB; ++j) tmask |= uint64_t(1) << ps.sbits[j];
+  const uint64_t cmask = ((ntiles << TB) - 1) & ~tmask;
+  const uint64_t dinc = deposit(G8);
 
   // The next tile is loaded into registers while the current one is
   // computed.  At a tile boundary the finished tile is read out of LDS, the
@@ -397,12 +490,18 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
   // wait for the prefetch placed after the stores would also wait for the
   // stores to reach memory.
   float4 v[IT];
-  uint64_t base = tile_base(tile);
+  uint64_t base = deposit(tile);
   if (tile < tend) {
 #pragma unroll
     for (int it = 0; it < IT; ++it) v[it] = ld_nt(a + (base | it_addr(it) | addr_t));
   }
   __syncthreads();  // operands staged
+  BlockScalars sa, sb;
+  BlockRegs ra, rb;
+  if (!SPLIT) {
+    sa = load_scalars(&blocks[ps.first_block], wave);
+    load_regs(ra, OPS);
+  }
   if (tile < tend) {
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
@@ -417,32 +516,46 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
 
   for (; tile < tend; tile += G8) {
     const uint64_t next = tile + G8;
-    const uint64_t nbase = tile_base(next);
+    const uint64_t nbase = ((base | ~cmask) + dinc) & cmask;
     if (next < tend) {
 #pragma unroll
       for (int it = 0; it < IT; ++it)
         v[it] = ld_nt(a + (nbase | it_addr(it) | addr_t));
     }
 
-    for (int b = 0; b < nb; ++b) {
-      const TileBlock *bl = &blocks[ps.first_block + b];
-      const int blk = OPS + b * STRIDE;
-      if (SPLIT) {
-        switch (bl->k) {
-          case 1: tp_block_split<1, NTH, TILE>(bl, blk); break;
-          case 2: tp_block_split<2, NTH, TILE>(bl, blk); break;
-          case 3: tp_block_split<3, NTH, TILE>(bl, blk); break;
-          default: tp_block_split<4, NTH, TILE>(bl, blk); break;
+    if (SPLIT) {
+      BlockScalars cur = load_scalars(&blocks[ps.first_block], wave);
+      // the first block's scalars are waited for here, once per tile: the
+      // compiler otherwise waits at the top of every block, for the next
+      // block's loads as well
+      __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0) only
+      for (int b = 0; b < nb; ++b) {
+        const TileBlock *bl = &blocks[ps.first_block + b];
+        const BlockScalars nxt = load_scalars(bl + min(1, nb - 1 - b), wave);
+        const int blk = OPS + b * STRIDE;
+        switch (cur.k) {
+          case 1: tp_block_split<1, NTH, TILE>(bl, cur, blk); break;
+          case 2: tp_block_split<2, NTH, TILE>(bl, cur, blk); break;
+          case 3: tp_block_split<3, NTH, TILE>(bl, cur, blk); break;
+          default: tp_block_split<4, NTH, TILE>(bl, cur, blk); break;
         }
-      } else {
-        switch (bl->k) {
-          case 1: tp_block<1, NTH, TILE>(bl, blk); break;
-          case 2: tp_block<2, NTH, TILE>(bl, blk); break;
-          case 3: tp_block<3, NTH, TILE>(bl, blk); break;
-          default: tp_block<4, NTH, TILE>(bl, blk); break;
-        }
+        if (cur.sync) __syncthreads();
+        cur = nxt;
       }
-      __syncthreads();
+    } else {
+      // two register sets taken in turn, so the values read one block ahead
+      // need no copies; (sa, ra) hold the pass's first block on entry
+      const TileBlock *fb = &blocks[ps.first_block];
+      for (int b = 0;;) {
+        tp_step<NTH, TILE>(fb, b, nb, OPS, wave, sa, ra, sb, rb);
+        if (++b == nb) {
+          sa = sb;
+          ra = rb;
+          break;
+        }
+        tp_step<NTH, TILE>(fb, b, nb, OPS, wave, sb, rb, sa, ra);
+        if (++b == nb) break;
+      }
     }
 
     float4 o[IT];
@@ -524,35 +637,121 @@ int tile_max_blocks(const qcs_hip_state *s) {
                                                       : QCS_HIP_TILE_MAX_BLOCKS;
 }
 
-// the block's descriptor: target positions, row offsets, lane tables
+// Low slot bit (LDS bank bit) that tile position p lands on under swz.
+static int bank_bit(int p) { return p < 5 ? p : p < 10 ? p - 5 : p - 10; }
+
+// Lane bits from `avail`: one position per bank bit where there is one
+// (lowest first), the rest the lowest positions left.  Returns the bank
+// conflict factor of the 32 columns a half-wave reads: 2^(5 - rank of their
+// bank bits), 1 when they fall in 32 distinct banks.
+static int pick_lanes(int avail, int lanes[5]) {
+  int n = 0, used = 0;
+  for (int c = 0; c < 5; ++c)
+    for (int p = 0; p < kTileBitsFixed; ++p)
+      if (((avail >> p) & 1) && bank_bit(p) == c) {
+        lanes[n++] = p;
+        used |= 1 << p;
+        break;
+      }
+  const int distinct = n;
+  for (int p = 0; p < kTileBitsFixed && n < 5; ++p)
+    if (((avail >> p) & 1) && !((used >> p) & 1)) lanes[n++] = p;
+  return 1 << (5 - distinct);
+}
+
+static int target_mask(const qcs_hip_pass &ps, const qcs_hip_block &blk) {
+  int m = 0;
+  for (int q = 0; q < blk.k; ++q) {
+    int pos = 0;
+    while (ps.bits[pos] != blk.targets[q]) ++pos;
+    m |= 1 << pos;
+  }
+  return m;
+}
+
+// value whose bit i sits at position pos[i], through swz, as a byte offset
+static int scatter_off(int v, const int *pos, int npos) {
+  int e = 0;
+  for (int i = 0; i < npos; ++i)
+    if ((v >> i) & 1) e |= 1 << pos[i];
+  return 4 * swz(e);
+}
+
+// The block's descriptor for the given wave bits (a mask of `nwb` tile
+// positions, none of them a target): row offsets, lane, wave and group
+// tables.  Lane bits are conflict-free where the remaining positions allow,
+// else the lowest ones.
 static void make_tile_block(const qcs_hip_pass &ps, const qcs_hip_block &blk, int wslot,
-                            TileBlock *tb) {
+                            int wmask, int nwb, bool sync, TileBlock *tb) {
   std::memset(tb, 0, sizeof(TileBlock));
   const int k = blk.k, R = 1 << k;
   tb->k = k;
   tb->wslot = wslot;
-  for (int q = 0; q < k; ++q) {
-    int pos = 0;
-    while (ps.bits[pos] != blk.targets[q]) ++pos;
-    tb->tpos[q] = pos;
-  }
-  for (int r = 0; r < R; ++r) {
-    int off = 0;
-    for (int q = 0; q < k; ++q)
-      if ((r >> q) & 1) off |= 1 << tb->tpos[q];
-    tb->rowoff[r] = 4 * swz(off);
-  }
-  for (int l = 0; l < 32; ++l) {
-    int c = l;
-    for (int q = 0; q < k; ++q) c = ins0(c, tb->tpos[q]);
-    tb->lanecol[l] = 4 * swz(c);
-  }
-  for (int h = 0; h < 2; ++h)
-    for (int q = 0; q < 4; ++q)
-      for (int j = 0; j < 2; ++j) {
-        const int r = 4 * q + 2 * h + j;
-        tb->outoff[h][2 * q + j] = r < R ? tb->rowoff[r] : 0;
+  tb->sync = sync ? 1 : 0;
+  int tpos[kTileKMax], wpos[kTileBitsFixed], lanes[5], jpos[kTileBitsFixed];
+  const int tmask = target_mask(ps, blk);
+  for (int q = 0, p = 0; p < kTileBitsFixed; ++p)
+    if ((tmask >> p) & 1) tpos[q++] = p;
+  int nw = 0;
+  for (int p = 0; p < kTileBitsFixed; ++p)
+    if ((wmask >> p) & 1) wpos[nw++] = p;
+  int avail = ((1 << kTileBitsFixed) - 1) & ~tmask & ~wmask;
+  (void)pick_lanes(avail, lanes);
+  for (int i = 0; i < 5; ++i) avail &= ~(1 << lanes[i]);
+  int nj = 0;
+  for (int p = 0; p < kTileBitsFixed; ++p)
+    if ((avail >> p) & 1) jpos[nj++] = p;
+  for (int r = 0; r < R; ++r) tb->rowoff[r] = scatter_off(r, tpos, k);
+  for (int l = 0; l < 32; ++l) tb->lanecol[l] = scatter_off(l, lanes, 5);
+  for (int w = 0; w < (1 << nwb); ++w) tb->wpart[w] = scatter_off(w, wpos, nwb);
+  for (int j = 0; j < (1 << nj); ++j) tb->jpart[j] = scatter_off(j, jpos, nj);
+}
+
+// Descriptors of one pass.  Blocks are taken in runs that share their wave
+// bits, and only the last block of a run is followed by a barrier: a run
+// grows while the union of its targets leaves nwb tile bits free.  The wave
+// bits of a run are the choice with the fewest bank conflicts over its
+// blocks (a shared choice can leave some of them 2-way conflicted: on this
+// kernel a barrier costs more than that).
+static void layout_pass(const qcs_hip_pass &ps, const qcs_hip_block *blocks, int nwb,
+                        TileBlock *tb) {
+  // conflict factor of every set of free positions, and the candidate wave
+  // bits of each width, computed once per process
+  constexpr int kSets = 1 << kTileBitsFixed;
+  static std::once_flag once;
+  static unsigned char factor[kSets];
+  static std::vector<int> masks[5];
+  std::call_once(once, [] {
+    int lanes[5];
+    for (int a = 0; a < kSets; ++a)
+      factor[a] = __builtin_popcount(unsigned(a)) >= 5 ? pick_lanes(a, lanes) : 0;
+    for (int m = 0; m < kSets; ++m)
+      if (__builtin_popcount(unsigned(m)) <= 4) masks[__builtin_popcount(unsigned(m))].push_back(m);
+  });
+  const int end = ps.first_block + ps.num_blocks;
+  std::vector<int> tm(size_t(ps.num_blocks));
+  for (int i = 0; i < ps.num_blocks; ++i) tm[i] = target_mask(ps, blocks[ps.first_block + i]);
+  for (int b = ps.first_block; b < end;) {
+    int u = tm[b - ps.first_block], e = b + 1;
+    for (; e < end; ++e) {
+      const int v = u | tm[e - ps.first_block];
+      if (__builtin_popcount(unsigned(v)) > kTileBitsFixed - nwb) break;
+      u = v;
+    }
+    int best = -1, best_cost = 0;
+    for (int m : masks[nwb]) {
+      if (m & u) continue;
+      int cost = 0;
+      for (int i = b; i < e; ++i) cost += factor[(kSets - 1) & ~tm[i - ps.first_block] & ~m];
+      if (best < 0 || cost < best_cost) {
+        best = m;
+        best_cost = cost;
       }
+    }
+    for (int i = b; i < e; ++i)
+      make_tile_block(ps, blocks[i], i, best, nwb, i == e - 1, &tb[i]);
+    b = e;
+  }
 }
 
 int tiles_apply(qcs_hip_state *s, const qcs_hip_pass *passes, int np,
@@ -569,15 +768,14 @@ int tiles_apply(qcs_hip_state *s, const qcs_hip_pass *passes, int np,
     return rc;
   TilePass *tp = reinterpret_cast<TilePass *>(stage);
   TileBlock *tb = reinterpret_cast<TileBlock *>(stage + pbytes);
+  const int nwb = s->tile_threads == 1024 ? 4 : 3;  // wave-index bits
   for (int p = 0; p < np; ++p) {
     std::memset(&tp[p], 0, sizeof(TilePass));
     tp[p].tb = passes[p].tile_bits;
     tp[p].nblocks = passes[p].num_blocks;
     tp[p].first_block = passes[p].first_block;
     for (int j = 0; j < passes[p].tile_bits; ++j) tp[p].sbits[j] = passes[p].bits[j];
-    for (int b = passes[p].first_block;
-         b < passes[p].first_block + passes[p].num_blocks; ++b)
-      make_tile_block(passes[p], blocks[b], b, &tb[b]);
+    layout_pass(passes[p], blocks, nwb, tb);
   }
   QCS_HIP_TRY(hipMemcpyAsync(s->tpass_dev, tp, pbytes, hipMemcpyHostToDevice,
                              s->stream));
