@@ -98,6 +98,7 @@ static char *dup_str(const char *s) {
 
 static void add_history(t_q_circuit *c, const char *name, int target,
                         int control, double param) {
+  if (c == NULL) return;
   if (c->history_size >= c->history_capacity) {
     int cap = c->history_capacity * 2;
     char **g = (char **)realloc(c->gate_history, cap * sizeof(char *));
