@@ -786,6 +786,9 @@ int tiles_apply(qcs_hip_state *s, const qcs_hip_pass *passes, int np,
   const bool split = s->tile_math == QCS_HIP_TILE_MATH_SPLIT_BF16;
   const double N = double(s->size);
   for (int p = 0; p < np; ++p) {
+    // the kernel runs its pass's first block unconditionally: an empty pass
+    // (which the planner does not make) would change nothing anyway
+    if (passes[p].num_blocks <= 0) continue;
     const uint64_t ntiles = s->size >> passes[p].tile_bits;
     double flops = 0.0;
     for (int b = passes[p].first_block;
