@@ -24,7 +24,10 @@ Beside it, in the same process:
     only (a bounded sample), 1 core; beside it the reference's OpenMP build
     and the oracle's race-fixed OpenMP loop on the box's cores.
 
-Usage: python bench.py [--steps K] [--warmup W] [--gpus 1]
+Other workloads (not the bench line): --workload batch (4096 circuits in one
+persistent launch), --workload qft (the 28-qubit QFT with swaps).
+
+Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload ...]
 """
 import argparse
 import json
@@ -63,10 +66,13 @@ def parse():
     p.add_argument("--no-alt", action="store_true",
                    help="skip the timing of the other fused modes")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
-    p.add_argument("--workload", choices=["fused30", "batch"], default="fused30",
+    p.add_argument("--workload", choices=["fused30", "batch", "qft"], default="fused30",
                    help="fused30: the 30-qubit fused-vs-plain line (default); batch: "
-                        "4096 independent 10-20 qubit circuits in one persistent launch")
+                        "4096 independent 10-20 qubit circuits in one persistent launch; "
+                        "qft: the 28-qubit QFT with swaps, plain complex128 and fused")
     p.add_argument("--batch-count", type=int, default=4096)
+    p.add_argument("--qft-qubits", type=int, default=28,
+                   help="qubits of the QFT workload (default 28)")
     p.add_argument("--batch-threads", type=int, default=512)
     p.add_argument("--no-tune", action="store_true",
                    help="skip the autotuner (default plans and kernel variants)")
@@ -203,6 +209,87 @@ class _Arrays:
     def __init__(self, a):
         self.op, self.tq, self.cq, self.par = (a[0].tolist(), a[1].tolist(),
                                                a[2].tolist(), a[3].tolist())
+
+
+def bench_qft(args):
+    """The 28-qubit QFT with final swaps (src/qcs.c:443-457 order: H(i), then
+    CPHASE(j -> i, pi/2^(j-i)) for j > i; 406 + 14 gates) on a basis state
+    |b>, b uniform from splitmix64 seed 1.  A step is: set |b>, apply the
+    gates, wait.  Plain complex128 (value) and fused FP32 tile programs beside
+    it; one 65536-amplitude range is checked against the closed form
+    exp(2 pi i rev(b) rev(y) / N) / sqrt(N)."""
+    import math
+    import numpy as np
+    import qcs_amd
+    from qcs_amd import circuits as C
+
+    hip, _ = qcs_amd.load()
+    if not hip.qcs_hip_device_available():
+        raise SystemExit("no HIP device")
+    n = args.qft_qubits
+    N = 1 << n
+    b = int(C.uniforms(1, 1)[0] * N)
+    ops = C.qft(n, swaps=True)
+    G = len(ops)
+
+    def rev(x):
+        r = np.zeros_like(x)
+        for i in range(n):
+            r |= ((x >> i) & 1) << (n - 1 - i)
+        return r
+
+    def run(options):
+        c = qcs_amd.Circuit(n, options)
+        st = c.device_state
+        for _ in range(args.warmup):
+            c.set_basis_state(b)
+            c.apply(ops)
+            c.flush()
+        hip.qcs_hip_timing_enable(st, 1)
+        hip.qcs_hip_device_sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            c.set_basis_state(b)
+            c.apply(ops)
+            c.flush()
+        return c, (time.perf_counter() - t0) / args.steps
+
+    plain, tp = run(qcs_amd.PLAIN_FP64)
+    st = plain.device_state
+    m1, n1, b1, _ = timing(hip, st, qcs_amd.K_1Q)
+    m2, n2, b2, _ = timing(hip, st, qcs_amd.K_CU)
+    off = 12345678 % (N - (1 << 16))
+    y = np.arange(off, off + (1 << 16), dtype=np.int64)
+    ref = np.exp(2j * np.pi * ((int(rev(np.array([b]))[0]) * rev(y)) % N) / N) / math.sqrt(N)
+    err = float(np.max(np.abs(plain.amplitudes(off, 1 << 16) - ref)))
+    fused, tf = run(qcs_amd.FUSED_FP32 | qcs_amd.FUSE_TILES)
+    omf, mx = fused.compare_with(plain)
+    blocks, passes = fused.get_num_fused_blocks(), fused.get_num_fused_passes()
+    fused.close()
+    plain.close()
+    print(f"bench: qft {n} q plain {tp * 1e3:.1f} ms, fused {tf * 1e3:.1f} ms per step",
+          file=sys.stderr, flush=True)
+    print(json.dumps({
+        "metric": f"amplitude updates/s, {n}-qubit QFT with swaps, complex128 plain",
+        "value": G * N / tp,
+        "unit": "amplitude updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": tp * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{n}-qubit QFT (reference gate order) with final swaps on "
+                               f"|b>, b = {b} (splitmix64 seed 1)",
+                   "qubits": n, "gates": G},
+        "gates_per_s": G / tp,
+        "closed_form_max_abs_err": err,
+        "k_1q": {"launches": n1, "avg_ms": m1 / max(n1, 1),
+                 "gbs": (b1 / max(n1, 1)) / (m1 / max(n1, 1) * 1e-3) / 1e9 if n1 else None},
+        "k_cu_and_swap": {"launches": n2, "avg_ms": m2 / max(n2, 1),
+                          "gbs_algorithmic": (b2 / max(n2, 1)) / (m2 / max(n2, 1) * 1e-3) / 1e9
+                          if n2 else None},
+        "fused_fp32_tiles": {"ms_per_step": tf * 1e3, "amp_updates_per_s": G * N / tf,
+                             "fused_blocks": blocks, "hbm_passes": passes,
+                             "one_minus_f": omf, "max_abs_err": mx},
+    }), flush=True)
 
 
 def bench_batch(args):
@@ -409,6 +496,9 @@ def main():
         return
     if args.workload == "batch":
         bench_batch(args)
+        return
+    if args.workload == "qft":
+        bench_qft(args)
         return
     import qcs_amd
     from qcs_amd import circuits as C
