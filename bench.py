@@ -264,7 +264,9 @@ def bench_qft(args):
     err = float(np.max(np.abs(plain.amplitudes(off, 1 << 16) - ref)))
     fused, tf = run(qcs_amd.FUSED_FP32 | qcs_amd.FUSE_TILES)
     omf, mx = fused.compare_with(plain)
-    blocks, passes = fused.get_num_fused_blocks(), fused.get_num_fused_passes()
+    runs = args.warmup + args.steps  # the counters add up over every run
+    blocks = fused.get_num_fused_blocks() // runs
+    passes = fused.get_num_fused_passes() // runs
     fused.close()
     plain.close()
     print(f"bench: qft {n} q plain {tp * 1e3:.1f} ms, fused {tf * 1e3:.1f} ms per step",
