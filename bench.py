@@ -66,13 +66,17 @@ def parse():
     p.add_argument("--no-alt", action="store_true",
                    help="skip the timing of the other fused modes")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
-    p.add_argument("--workload", choices=["fused30", "batch", "qft"], default="fused30",
+    p.add_argument("--workload", choices=["fused30", "batch", "qft", "observe"],
+                   default="fused30",
                    help="fused30: the 30-qubit fused-vs-plain line (default); batch: "
                         "4096 independent 10-20 qubit circuits in one persistent launch; "
-                        "qft: the 28-qubit QFT with swaps, plain complex128 and fused")
+                        "qft: the 28-qubit QFT with swaps, plain complex128 and fused; "
+                        "observe: the observation kernels on a 30-qubit complex128 state")
     p.add_argument("--batch-count", type=int, default=4096)
     p.add_argument("--qft-qubits", type=int, default=28,
                    help="qubits of the QFT workload (default 28)")
+    p.add_argument("--obs-qubits", type=int, default=30,
+                   help="qubits of the observe workload (default 30)")
     p.add_argument("--batch-threads", type=int, default=512)
     p.add_argument("--no-tune", action="store_true",
                    help="skip the autotuner (default plans and kernel variants)")
@@ -294,6 +298,99 @@ def bench_qft(args):
     }), flush=True)
 
 
+def bench_observe(args):
+    """The observation side of the plain form on a 30-qubit complex128 state
+    (16 GiB; H on every qubit, then one layered Rz-Ry-Rz + CNOT layer): norm
+    reduction, renormalisation, argmax, one-qubit measurement, 1024-shot
+    sampling and Grover rounds, each through the C ABI, timed with a device
+    synchronisation around it (median of --steps runs), against the
+    algorithmic bytes of each operation: N*s read for a reduction, 3*N*s for
+    reduce + rescale, 2*N*s for a measurement (p0 on one half, collapse and
+    rescale), N*s + 2*N*8 + shots*log2(N)*8 for sampling, 3*N*s per textbook
+    Grover round (its phase flip touches one amplitude).  moved_bytes is what
+    this implementation reads and writes where that differs: sampling never
+    writes the CDF (chunk sums, then one 4096-amplitude chunk per shot)."""
+    import ctypes
+    import statistics
+    import numpy as np
+    import qcs_amd
+    from qcs_amd import circuits as C
+
+    hip, _ = qcs_amd.load()
+    if not hip.qcs_hip_device_available():
+        raise SystemExit("no HIP device")
+    n, s = args.obs_qubits, 16
+    N = float(1 << n)
+    c = qcs_amd.Circuit(n)
+    c.apply(C.layered(n, 1, args.seed))
+    st = c.device_state
+    d = ctypes.c_double()
+    l = ctypes.c_long()
+    o = ctypes.c_int()
+    shots = 1024
+    u = (ctypes.c_double * shots)(*np.random.default_rng(0).random(shots))
+    idx = (ctypes.c_long * shots)()
+
+    def ok(rc):
+        if rc != 0:
+            raise RuntimeError(hip.qcs_hip_last_error())
+
+    amp = (ctypes.c_double * 2)()
+
+    def perturb():
+        # one amplitude changed (untimed), so the norm is not 1 and
+        # normalize rescales the state
+        hip.qcs_hip_read(st, 0, 1, amp)
+        amp[0] *= 1.5
+        ok(hip.qcs_hip_write(st, 0, 1, amp))
+
+    # name: (call, algorithmic bytes, bytes this implementation moves, setup)
+    chunk = 4096 * s  # a shot scans one 4096-amplitude chunk
+    ops = {
+        "norm_sq": (lambda: ok(hip.qcs_hip_norm_sq(st, ctypes.byref(d))), N * s, N * s, None),
+        "normalize": (lambda: ok(hip.qcs_hip_normalize(st, None)), 3 * N * s, 3 * N * s,
+                      perturb),
+        "argmax": (lambda: ok(hip.qcs_hip_argmax(st, ctypes.byref(l))), N * s, N * s, None),
+        "measure_qubit_7": (lambda: ok(hip.qcs_hip_measure(st, 7, 0.25, ctypes.byref(o))),
+                            2 * N * s, 2 * N * s, None),
+        "sample_1024_shots": (lambda: ok(hip.qcs_hip_sample(st, shots, u, idx)),
+                              N * s + 2 * N * 8 + shots * n * 8, N * s + shots * chunk, None),
+        "grover_round": (lambda: ok(hip.qcs_hip_grover(st, 12345, 1)), 3 * N * s, 3 * N * s,
+                         None),
+        "probability": (lambda: ok(hip.qcs_hip_probability(st, 12345, ctypes.byref(d))),
+                        2 * s, 2 * s, None),
+    }
+    out = {}
+    for name, (fn, nbytes, moved, setup) in ops.items():
+        times = []
+        for i in range(args.warmup + args.steps):
+            if setup:
+                setup()
+            hip.qcs_hip_device_sync()
+            t0 = time.perf_counter()
+            fn()
+            ok(hip.qcs_hip_sync(st))
+            if i >= args.warmup:
+                times.append(time.perf_counter() - t0)
+        t = statistics.median(times)
+        out[name] = {"ms": t * 1e3, "algorithmic_bytes": nbytes,
+                     "gbs": nbytes / t / 1e9, "hbm_frac": nbytes / t / 1e9 / HBM_PEAK_GBS,
+                     "moved_bytes": moved, "moved_gbs": moved / t / 1e9}
+        print(f"bench: observe {name} {t * 1e3:.3f} ms", file=sys.stderr, flush=True)
+    c.close()
+    print(json.dumps({
+        "metric": f"observation kernels on a {n}-qubit complex128 state (GB/s of algorithmic bytes)",
+        "value": out["norm_sq"]["gbs"], "unit": "GB/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": out["norm_sq"]["ms"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{n}-qubit complex128 state after H on every qubit and one "
+                               f"layered layer (seed {args.seed}); one call per timed run",
+                   "qubits": n},
+        "ops": out,
+    }), flush=True)
+
+
 def bench_batch(args):
     """4096 independent layered circuits (n ~ U[10,20], depth ~ U[5,40],
     splitmix64 seed 3; even circuits complex128 plain, odd fused complex64),
@@ -501,6 +598,9 @@ def main():
         return
     if args.workload == "qft":
         bench_qft(args)
+        return
+    if args.workload == "observe":
+        bench_observe(args)
         return
     import qcs_amd
     from qcs_amd import circuits as C
