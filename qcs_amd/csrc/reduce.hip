@@ -85,13 +85,14 @@ __global__ __launch_bounds__(kThreads) void k_final(const double *partial,
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_scale(typename Amp<T>::type *a,
                                                     uint64_t n, double f) {
-  const uint64_t stride = uint64_t(gridDim.x) * kThreads;
-  for (uint64_t i = uint64_t(blockIdx.x) * kThreads + threadIdx.x; i < n;
-       i += stride) {
-    typename Amp<T>::type v = a[i];
+  const uint64_t nvb = ceil_div(n, uint64_t(kThreads));
+  for (uint64_t vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
+    const uint64_t i = xcd_order(vb, nvb) * kThreads + threadIdx.x;
+    if (i >= n) continue;
+    typename Amp<T>::type v = ld_stream(a + i);
     v.x = T(double(v.x) * f);
     v.y = T(double(v.y) * f);
-    a[i] = v;
+    st_stream(a + i, v);
   }
 }
 
@@ -103,13 +104,14 @@ __global__ __launch_bounds__(kThreads) void k_scale_dev(
   const double v = *ns;
   if (!(v > 1e-12 && v != 1.0)) return;
   const double f = 1.0 / sqrt(v);
-  const uint64_t stride = uint64_t(gridDim.x) * kThreads;
-  for (uint64_t i = uint64_t(blockIdx.x) * kThreads + threadIdx.x; i < n;
-       i += stride) {
-    typename Amp<T>::type x = a[i];
+  const uint64_t nvb = ceil_div(n, uint64_t(kThreads));
+  for (uint64_t vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
+    const uint64_t i = xcd_order(vb, nvb) * kThreads + threadIdx.x;
+    if (i >= n) continue;
+    typename Amp<T>::type x = ld_stream(a + i);
     x.x = T(double(x.x) * f);
     x.y = T(double(x.y) * f);
-    a[i] = x;
+    st_stream(a + i, x);
   }
 }
 
@@ -119,20 +121,21 @@ template <typename T>
 __global__ __launch_bounds__(kThreads) void k_collapse(
     typename Amp<T>::type *a, uint64_t n, int q, int outcome, double f,
     int do_scale) {
-  const uint64_t stride = uint64_t(gridDim.x) * kThreads;
-  for (uint64_t i = uint64_t(blockIdx.x) * kThreads + threadIdx.x; i < n;
-       i += stride) {
+  const uint64_t nvb = ceil_div(n, uint64_t(kThreads));
+  for (uint64_t vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
+    const uint64_t i = xcd_order(vb, nvb) * kThreads + threadIdx.x;
+    if (i >= n) continue;
     const int bit = int((i >> q) & 1);
     typename Amp<T>::type v;
     if (bit != outcome) {
       v.x = T(0);
       v.y = T(0);
-      a[i] = v;
+      st_stream(a + i, v);
     } else if (do_scale) {
-      v = a[i];
+      v = ld_stream(a + i);
       v.x = T(double(v.x) * f);
       v.y = T(double(v.y) * f);
-      a[i] = v;
+      st_stream(a + i, v);
     }
   }
 }
@@ -369,15 +372,17 @@ __global__ __launch_bounds__(kThreads) void k_gupdate(typename Amp<T>::type *a,
                                                       double *partial) {
   __shared__ double lds[kThreads / 64];
   const double tr = two_mu[0], ti = two_mu[1];
-  const uint64_t stride = uint64_t(gridDim.x) * kThreads;
   double sr = 0.0, si = 0.0;
-  for (uint64_t i = uint64_t(blockIdx.x) * kThreads + threadIdx.x; i < n;
-       i += stride) {
-    typename Amp<T>::type v = a[i];
+  // XCD-ordered virtual blocks: each XCD sweeps one contiguous range
+  const uint64_t nvb = ceil_div(n, uint64_t(kThreads));
+  for (uint64_t vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
+    const uint64_t i = xcd_order(vb, nvb) * kThreads + threadIdx.x;
+    if (i >= n) continue;
+    typename Amp<T>::type v = ld_stream(a + i);
     const double sg = (i == m) ? -1.0 : 1.0;
     v.x = T(tr - sg * double(v.x));
     v.y = T(ti - sg * double(v.y));
-    a[i] = v;
+    st_stream(a + i, v);
     sr += sg * double(v.x);
     si += sg * double(v.y);
   }
@@ -514,6 +519,13 @@ static unsigned stream_grid(uint64_t n) {
   return unsigned(b < 8192 ? b : 8192);
 }
 
+// element-wise sweeps: one amplitude per thread over XCD-ordered virtual
+// blocks, up to 2^20 work-groups (the gate kernels' launch shape)
+static unsigned elem_grid(uint64_t n) {
+  const uint64_t b = ceil_div(n, kThreads), cap = uint64_t(1) << 20;
+  return unsigned(b < cap ? b : cap);
+}
+
 template <typename T>
 static int norm_partials(qcs_hip_state *s, int q, unsigned nb) {
   typedef typename Amp<T>::type A;
@@ -551,11 +563,11 @@ static int scale_impl(qcs_hip_state *s, double f, const double *dev_ns) {
   timing_begin(s, QCS_HIP_K_OTHER, 2.0 * double(s->size) * sizeof(A),
                2.0 * s->size, &tl);
   if (dev_ns)
-    hipLaunchKernelGGL(k_scale_dev<T>, dim3(stream_grid(s->size)),
+    hipLaunchKernelGGL(k_scale_dev<T>, dim3(elem_grid(s->size)),
                        dim3(kThreads), 0, s->stream, static_cast<A *>(s->vec),
                        s->size, dev_ns);
   else
-    hipLaunchKernelGGL(k_scale<T>, dim3(stream_grid(s->size)), dim3(kThreads),
+    hipLaunchKernelGGL(k_scale<T>, dim3(elem_grid(s->size)), dim3(kThreads),
                        0, s->stream, static_cast<A *>(s->vec), s->size, f);
   int rc = launch_check("k_scale");
   if (rc) return rc;
@@ -596,7 +608,7 @@ static int collapse_impl(qcs_hip_state *s, int q, int outcome, double f,
   typedef typename Amp<T>::type A;
   TimedLaunch tl;
   timing_begin(s, QCS_HIP_K_OTHER, 1.5 * double(s->size) * sizeof(A), 0.0, &tl);
-  hipLaunchKernelGGL(k_collapse<T>, dim3(stream_grid(s->size)), dim3(kThreads),
+  hipLaunchKernelGGL(k_collapse<T>, dim3(elem_grid(s->size)), dim3(kThreads),
                      0, s->stream, static_cast<A *>(s->vec), s->size, q,
                      outcome, f, do_scale);
   int rc = launch_check("k_collapse");
