@@ -26,22 +26,29 @@ template <typename T>
 __global__ __launch_bounds__(kThreads) void k_norm_partial(
     const typename Amp<T>::type *a, uint64_t n, int q, double *partial) {
   __shared__ double lds[kThreads / 64];
-  const uint64_t stride = uint64_t(gridDim.x) * kThreads;
   double s0 = 0.0, s1 = 0.0;
   const uint64_t qb = q >= 0 ? (uint64_t(1) << q) : 0;
-  uint64_t i = uint64_t(blockIdx.x) * kThreads + threadIdx.x;
-  for (; i + 3 * stride < n; i += 4 * stride) {
-    const typename Amp<T>::type v0 = a[i], v1 = a[i + stride],
-                                v2 = a[i + 2 * stride], v3 = a[i + 3 * stride];
-    const double p0 = norm2(v0), p1 = norm2(v1), p2 = norm2(v2),
-                 p3 = norm2(v3);
-    if (i & qb) s1 += p0; else s0 += p0;
-    if ((i + stride) & qb) s1 += p1; else s0 += p1;
-    if ((i + 2 * stride) & qb) s1 += p2; else s0 += p2;
-    if ((i + 3 * stride) & qb) s1 += p3; else s0 += p3;
+  // XCD-ordered virtual blocks (each XCD reads one contiguous range), four
+  // loads in flight per thread
+  const uint64_t nvb = ceil_div(n, uint64_t(kThreads)), G = gridDim.x;
+  auto at = [&](uint64_t vb) { return xcd_order(vb, nvb) * kThreads + threadIdx.x; };
+  uint64_t vb = blockIdx.x;
+  for (; vb + 3 * G < nvb; vb += 4 * G) {
+    const uint64_t i0 = at(vb), i1 = at(vb + G), i2 = at(vb + 2 * G), i3 = at(vb + 3 * G);
+    // a virtual block other than the last is full; the last is vb + 3G at most
+    const typename Amp<T>::type v0 = ld_stream(a + i0), v1 = ld_stream(a + i1),
+                                v2 = ld_stream(a + i2);
+    const double p0 = norm2(v0), p1 = norm2(v1), p2 = norm2(v2);
+    const double p3 = i3 < n ? norm2(ld_stream(a + i3)) : 0.0;
+    if (i0 & qb) s1 += p0; else s0 += p0;
+    if (i1 & qb) s1 += p1; else s0 += p1;
+    if (i2 & qb) s1 += p2; else s0 += p2;
+    if (i3 & qb) s1 += p3; else s0 += p3;
   }
-  for (; i < n; i += stride) {
-    const double p = norm2(a[i]);
+  for (; vb < nvb; vb += G) {
+    const uint64_t i = at(vb);
+    if (i >= n) continue;
+    const double p = norm2(ld_stream(a + i));
     if (i & qb) s1 += p; else s0 += p;
   }
   s0 = block_sum(s0, lds);
@@ -152,16 +159,32 @@ __global__ __launch_bounds__(kThreads) void k_argmax_partial(
     const typename Amp<T>::type *a, uint64_t n, double *partial) {
   __shared__ double lp[kThreads];
   __shared__ uint64_t li[kThreads];
-  const uint64_t stride = uint64_t(gridDim.x) * kThreads;
   double bp = -1.0;
   uint64_t bi = ~uint64_t(0);
-  for (uint64_t i = uint64_t(blockIdx.x) * kThreads + threadIdx.x; i < n;
-       i += stride) {
-    const double p = norm2(a[i]);
-    if (p > bp) {  // strict: the first maximum in index order wins; NaN skipped
+  // XCD-ordered virtual blocks, four loads in flight; a thread does not
+  // visit its indices in increasing order, so ties go to the smaller index
+  // explicitly (the first maximum in index order wins; NaN is skipped)
+  const uint64_t nvb = ceil_div(n, uint64_t(kThreads)), G = gridDim.x;
+  auto at = [&](uint64_t vb) { return xcd_order(vb, nvb) * kThreads + threadIdx.x; };
+  auto take = [&](double p, uint64_t i) {
+    if (better(p, i, bp, bi)) {
       bp = p;
       bi = i;
     }
+  };
+  uint64_t vb = blockIdx.x;
+  for (; vb + 3 * G < nvb; vb += 4 * G) {
+    const uint64_t i0 = at(vb), i1 = at(vb + G), i2 = at(vb + 2 * G), i3 = at(vb + 3 * G);
+    const double p0 = norm2(ld_stream(a + i0)), p1 = norm2(ld_stream(a + i1)),
+                 p2 = norm2(ld_stream(a + i2));
+    take(p0, i0);
+    take(p1, i1);
+    take(p2, i2);
+    if (i3 < n) take(norm2(ld_stream(a + i3)), i3);
+  }
+  for (; vb < nvb; vb += G) {
+    const uint64_t i = at(vb);
+    if (i < n) take(norm2(ld_stream(a + i)), i);
   }
   lp[threadIdx.x] = bp;
   li[threadIdx.x] = bi;
@@ -247,27 +270,57 @@ __global__ __launch_bounds__(kThreads) void k_chunk_tot(
 }
 
 // One work-group: excl[c] (in place over tot) and inc[c] = excl[c] + tot[c].
+// The additions run in a fixed order (each thread's segment left to right,
+// then the segment sums left to right), which the batch kernel's sampling
+// shares; the loads are issued 16 at a time ahead of the additions, so the
+// sequential chains wait on arithmetic rather than on memory.
 __global__ __launch_bounds__(1024) void k_chunk_scan(double *tot, double *inc,
                                                      uint64_t nchunks) {
+  constexpr int U = 16;
   __shared__ double base[1024];
   const uint64_t per = ceil_div(nchunks, 1024);
   const uint64_t b = uint64_t(threadIdx.x) * per;
   const uint64_t e = b + per < nchunks ? b + per : nchunks;
   double s = 0.0;
-  for (uint64_t c = b; c < e; ++c) s = s + tot[c];
+  uint64_t c = b;
+  for (; c + U <= e; c += U) {
+    double v[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) v[j] = tot[c + j];
+#pragma unroll
+    for (int j = 0; j < U; ++j) s = s + v[j];
+  }
+  for (; c < e; ++c) s = s + tot[c];
   base[threadIdx.x] = s;
   __syncthreads();
   if (threadIdx.x == 0) {
     double run = 0.0;
-    for (int t = 0; t < 1024; ++t) {
-      const double v = base[t];
-      base[t] = run;
-      run = run + v;
+    for (int t0 = 0; t0 < 1024; t0 += U) {
+      double v[U];
+#pragma unroll
+      for (int j = 0; j < U; ++j) v[j] = base[t0 + j];
+#pragma unroll
+      for (int j = 0; j < U; ++j) {
+        base[t0 + j] = run;
+        run = run + v[j];
+      }
     }
   }
   __syncthreads();
   double run = base[threadIdx.x];
-  for (uint64_t c = b; c < e; ++c) {
+  c = b;
+  for (; c + U <= e; c += U) {
+    double v[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) v[j] = tot[c + j];
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      tot[c + j] = run;
+      run = run + v[j];
+      inc[c + j] = run;
+    }
+  }
+  for (; c < e; ++c) {
     const double t = tot[c];
     tot[c] = run;
     run = run + t;
@@ -330,11 +383,12 @@ template <typename T>
 __global__ __launch_bounds__(kThreads) void k_gsum_partial(
     const typename Amp<T>::type *a, uint64_t n, uint64_t m, double *partial) {
   __shared__ double lds[kThreads / 64];
-  const uint64_t stride = uint64_t(gridDim.x) * kThreads;
   double sr = 0.0, si = 0.0;
-  for (uint64_t i = uint64_t(blockIdx.x) * kThreads + threadIdx.x; i < n;
-       i += stride) {
-    const typename Amp<T>::type v = a[i];
+  const uint64_t nvb = ceil_div(n, uint64_t(kThreads));
+  for (uint64_t vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
+    const uint64_t i = xcd_order(vb, nvb) * kThreads + threadIdx.x;
+    if (i >= n) continue;
+    const typename Amp<T>::type v = ld_stream(a + i);
     const double sg = (i == m) ? -1.0 : 1.0;
     sr += sg * double(v.x);
     si += sg * double(v.y);

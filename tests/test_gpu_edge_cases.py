@@ -120,3 +120,24 @@ def test_batch_argument_checks(qlib):
         q.qc_destroy(c)
         q.qc_destroy(plain)
         q.qc_batch_destroy(b)
+
+
+def test_argmax_ties_go_to_the_first_index(qlib):
+    """qc_find_most_likely_state keeps the first maximum in index order
+    (strict >, src/qcs.c:464-478), also when the equal maxima lie in
+    different work-groups' ranges and the state is uniform."""
+    n = 22
+    N = 1 << n
+    with qcs_amd.Circuit(n) as c:
+        for q in range(n):
+            c.h(q)
+        assert c.find_most_likely_state() == 0
+        for i, j in ((5, N - 3), (123457, 3000001), (N // 2 + 7, N - 1)):
+            amps = np.zeros(N, np.complex128)
+            amps[[i, j]] = 1 / math.sqrt(2)
+            c.set_amplitudes(amps)
+            assert c.find_most_likely_state() == i
+            amps[j] = -1j / math.sqrt(2)          # equal |a|^2, other phase
+            amps[N - 2] = 0.5                      # smaller
+            c.set_amplitudes(amps)
+            assert c.find_most_likely_state() == i
