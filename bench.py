@@ -215,6 +215,75 @@ class _Arrays:
                                                a[2].tolist(), a[3].tolist())
 
 
+def _reflib():
+    """the reference's sequential library built from its sources (oracle/_ref),
+    or None where it was not built"""
+    try:
+        from oracle.oracle import RefLib
+        return RefLib("libqcsref.so")
+    except (FileNotFoundError, OSError, ImportError):
+        return None
+
+
+def ref_qft_baseline(n=18):
+    """The reference's qc_quantum_fourier_transform (no swaps; its CPHASE as
+    written) on a fresh n-qubit circuit, one call, 1 core."""
+    ref = _reflib()
+    if ref is None:
+        return None
+    c = ref.create(n)
+    t0 = time.perf_counter()
+    ref.L.qc_quantum_fourier_transform(c)
+    dt = time.perf_counter() - t0
+    g = ref.L.qc_get_num_gates(c)
+    ref.L.qc_destroy(c)
+    return {"value": g * float(1 << n) / dt, "unit": "amplitude updates/s", "cores": 1,
+            "kind": "reference",
+            "sample": f"qc_quantum_fourier_transform on {n} qubits ({g} gates, no swaps), "
+                      f"{dt:.1f} s, reference sequential build",
+            "gates_per_s": g / dt, "cpu": cpu_model()}
+
+
+def ref_observe_baseline(n=20, shots=1024):
+    """The reference's observation calls on an n-qubit state (H on every
+    qubit), 1 core, as GB/s of the same algorithmic bytes as the GPU line."""
+    ref = _reflib()
+    if ref is None:
+        return None
+    import ctypes
+    s = 16
+    N = float(1 << n)
+    c = ref.create(n)
+    for q in range(n):
+        ref.L.qc_h(c, q)
+    out = {}
+
+    def timed(name, fn, nbytes):
+        t0 = time.perf_counter()
+        fn()
+        dt = time.perf_counter() - t0
+        out[name] = {"ms": dt * 1e3, "gbs": nbytes / dt / 1e9}
+
+    hist = (ctypes.c_int * (1 << n))()
+    timed("argmax", lambda: ref.L.qc_find_most_likely_state(c), N * s)
+    timed("sample_1024_shots", lambda: ref.L.qc_run_shots(c, shots, hist),
+          N * s + 2 * N * 8 + shots * n * 8)
+    timed("measure_qubit_7", lambda: ref.L.qc_measure(c, 7), 2 * N * s)
+    ref.L.qc_destroy(c)
+    g = ref.create(16)
+    t0 = time.perf_counter()
+    ref.L.qc_grover_search(g, 12345)
+    dt = time.perf_counter() - t0
+    rounds = 201  # floor(pi/4 * sqrt(2^16))
+    out["grover_round_16q"] = {"ms": dt / rounds * 1e3,
+                               "gbs": 3 * float(1 << 16) * s * rounds / dt / 1e9}
+    ref.L.qc_destroy(g)
+    return {"value": out["argmax"]["gbs"], "unit": "GB/s", "cores": 1, "kind": "reference",
+            "sample": f"{n}-qubit state (Grover: 16 qubits, {rounds} rounds), one call each, "
+                      "reference sequential build",
+            "ops": out, "cpu": cpu_model()}
+
+
 def bench_qft(args):
     """The 28-qubit QFT with final swaps (src/qcs.c:443-457 order: H(i), then
     CPHASE(j -> i, pi/2^(j-i)) for j > i; 406 + 14 gates) on a basis state
@@ -295,6 +364,7 @@ def bench_qft(args):
         "fused_fp32_tiles": {"ms_per_step": tf * 1e3, "amp_updates_per_s": G * N / tf,
                              "fused_blocks": blocks, "hbm_passes": passes,
                              "one_minus_f": omf, "max_abs_err": mx},
+        "cpu_baseline": None if args.no_cpu else ref_qft_baseline(),
     }), flush=True)
 
 
@@ -390,6 +460,7 @@ def bench_observe(args):
                                f"layered layer (seed {args.seed}); one call per timed run",
                    "qubits": n},
         "ops": out,
+        "cpu_baseline": None if args.no_cpu else ref_observe_baseline(),
     }), flush=True)
 
 
