@@ -244,7 +244,7 @@ def ref_qft_baseline(n=18):
             "gates_per_s": g / dt, "cpu": cpu_model()}
 
 
-def ref_observe_baseline(n=20, shots=1024):
+def ref_observe_baseline(n=20, shots=1024, grover_n=16):
     """The reference's observation calls on an n-qubit state (H on every
     qubit), 1 core, as GB/s of the same algorithmic bytes as the GPU line."""
     ref = _reflib()
@@ -266,20 +266,21 @@ def ref_observe_baseline(n=20, shots=1024):
 
     hist = (ctypes.c_int * (1 << n))()
     timed("argmax", lambda: ref.L.qc_find_most_likely_state(c), N * s)
-    timed("sample_1024_shots", lambda: ref.L.qc_run_shots(c, shots, hist),
+    timed(f"sample_{shots}_shots", lambda: ref.L.qc_run_shots(c, shots, hist),
           N * s + 2 * N * 8 + shots * n * 8)
     timed("measure_qubit_7", lambda: ref.L.qc_measure(c, 7), 2 * N * s)
     ref.L.qc_destroy(c)
-    g = ref.create(16)
+    g = ref.create(grover_n)
     t0 = time.perf_counter()
-    ref.L.qc_grover_search(g, 12345)
+    ref.L.qc_grover_search(g, 12345 % (1 << grover_n))
     dt = time.perf_counter() - t0
-    rounds = 201  # floor(pi/4 * sqrt(2^16))
-    out["grover_round_16q"] = {"ms": dt / rounds * 1e3,
-                               "gbs": 3 * float(1 << 16) * s * rounds / dt / 1e9}
+    # the reference's iteration count, floor(pi/4 * sqrt(2^n)) (src/q_utils.c:126-134)
+    rounds = int(math.pi / 4.0 * math.sqrt(float(1 << grover_n)))
+    out[f"grover_round_{grover_n}q"] = {
+        "ms": dt / rounds * 1e3, "gbs": 3 * float(1 << grover_n) * s * rounds / dt / 1e9}
     ref.L.qc_destroy(g)
     return {"value": out["argmax"]["gbs"], "unit": "GB/s", "cores": 1, "kind": "reference",
-            "sample": f"{n}-qubit state (Grover: 16 qubits, {rounds} rounds), one call each, "
+            "sample": f"{n}-qubit state (Grover: {grover_n} qubits, {rounds} rounds), one call each, "
                       "reference sequential build",
             "ops": out, "cpu": cpu_model()}
 
