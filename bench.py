@@ -66,12 +66,13 @@ def parse():
     p.add_argument("--no-alt", action="store_true",
                    help="skip the timing of the other fused modes")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
-    p.add_argument("--workload", choices=["fused30", "batch", "qft", "observe"],
+    p.add_argument("--workload", choices=["fused30", "batch", "qft", "observe", "config5"],
                    default="fused30",
                    help="fused30: the 30-qubit fused-vs-plain line (default); batch: "
                         "4096 independent 10-20 qubit circuits in one persistent launch; "
                         "qft: the 28-qubit QFT with swaps, plain complex128 and fused; "
-                        "observe: the observation kernels on a 30-qubit complex128 state")
+                        "observe: the observation kernels on a 30-qubit complex128 state; "
+                        "config5: 33-qubit Grover (complex128) and the 34-qubit fused circuit")
     p.add_argument("--batch-count", type=int, default=4096)
     p.add_argument("--qft-qubits", type=int, default=28,
                    help="qubits of the QFT workload (default 28)")
@@ -465,6 +466,83 @@ def bench_observe(args):
     }), flush=True)
 
 
+def bench_config5(args):
+    """The largest single-GPU states, one timed run each (after one warmup):
+    (a) 33 qubits complex128 (128 GiB): Grover for the marked index from
+    splitmix64 seed 4, truncated to 64 rounds, checked against the analytic
+    amplitude sin((2k+1) asin(2^-n/2)); (b) 34 qubits complex64 (128 GiB),
+    freed (a) first: the depth-20 layered circuit (2,734 gates) as fused FP32
+    tile programs, one flush."""
+    import math
+    import qcs_amd
+    from qcs_amd import circuits as C
+
+    hip, _ = qcs_amd.load()
+    if not hip.qcs_hip_device_available():
+        raise SystemExit("no HIP device")
+    out = {}
+    n, k = 33, 64
+    m = int(C.uniforms(4, 1)[0] * (1 << n))
+    want = math.sin((2 * k + 1) * math.asin(1.0 / math.sqrt(2.0 ** n))) ** 2
+    with qcs_amd.Circuit(n) as g:
+        st = g.device_state
+        for _ in range(2):  # warmup, then the timed run
+            g.set_basis_state(0)
+            hip.qcs_hip_device_sync()
+            t0 = time.perf_counter()
+            for q in range(n):
+                g.h(q)
+            g.flush()
+            t_h = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            # the rounds of qc_grover_search_iters, on the device
+            if hip.qcs_hip_grover(st, m, k) != 0 or hip.qcs_hip_sync(st) != 0:
+                raise RuntimeError(hip.qcs_hip_last_error())
+            t_r = time.perf_counter() - t0
+        p = g.get_probability_l(m)
+        N, sb = float(1 << n), 16.0
+        out["grover_33q_c128"] = {
+            "h_layer_ms": t_h * 1e3, "rounds": k, "rounds_ms": t_r * 1e3,
+            "ms_per_round": t_r / k * 1e3,
+            # the reference's round: a sum, then a read+write update (3*N*s);
+            # here the update also forms the next round's sum (2*N*s moved)
+            "round_gbs": 3 * N * sb / (t_r / k) / 1e9,
+            "round_moved_gbs": 2 * N * sb / (t_r / k) / 1e9,
+            "round_moved_hbm_frac": 2 * N * sb / (t_r / k) / 1e9 / HBM_PEAK_GBS,
+            "marked": m, "p_marked": p, "p_marked_analytic": want,
+            "abs_err": abs(p - want)}
+    t = t_h + t_r
+    print(f"bench: config5 grover 33 q {t * 1e3:.0f} ms", file=sys.stderr, flush=True)
+    n, depth = 34, 20
+    ops = C.layered(n, depth, 2)
+    G = len(ops)
+    with qcs_amd.Circuit(n, qcs_amd.FUSED_FP32 | qcs_amd.FUSE_TILES) as f:
+        times = []
+        for _ in range(2):
+            f.set_basis_state(0)
+            hip.qcs_hip_device_sync()
+            t0 = time.perf_counter()
+            f.apply(ops)
+            f.flush()
+            times.append(time.perf_counter() - t0)
+        t = times[-1]
+        out["fused_34q_tiles"] = {
+            "ms": t * 1e3, "gates": G, "amp_updates_per_s": G * float(1 << n) / t,
+            "fused_blocks": f.get_num_fused_blocks() // 2,
+            "hbm_passes": f.get_num_fused_passes() // 2, "norm_sq": f.norm_sq()}
+    print(f"bench: config5 fused 34 q {t * 1e3:.0f} ms", file=sys.stderr, flush=True)
+    print(json.dumps({
+        "metric": "33-qubit complex128 Grover and 34-qubit fused FP32 (largest single-GPU states)",
+        "value": out["fused_34q_tiles"]["amp_updates_per_s"], "unit": "amplitude updates/s",
+        "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": out["fused_34q_tiles"]["ms"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64 (Grover), f32 (fused)",
+        "data": "synthetic",
+        "config": {"workload": "33 q Grover, marked from splitmix64 seed 4, 64 rounds; 34 q "
+                               "layered depth 20 (seed 2) as fused FP32 tile programs"},
+        **out}), flush=True)
+
+
 def bench_batch(args):
     """4096 independent layered circuits (n ~ U[10,20], depth ~ U[5,40],
     splitmix64 seed 3; even circuits complex128 plain, odd fused complex64),
@@ -675,6 +753,9 @@ def main():
         return
     if args.workload == "observe":
         bench_observe(args)
+        return
+    if args.workload == "config5":
+        bench_config5(args)
         return
     import qcs_amd
     from qcs_amd import circuits as C

@@ -23,7 +23,7 @@ def test_more_than_one_gpu_is_refused():
     assert "one GPU only" in r.stderr
 
 
-@pytest.mark.parametrize("workload", ["fused30", "batch", "qft", "observe"])
+@pytest.mark.parametrize("workload", ["fused30", "batch", "qft", "observe", "config5"])
 def test_workloads_need_a_device(workload):
     import qcs_amd
     hip, _ = qcs_amd.load()
