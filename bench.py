@@ -383,7 +383,10 @@ def bench_observe(args):
     this implementation reads and writes where that differs: sampling never
     writes the CDF (chunk sums, then one 4096-amplitude chunk per shot); a
     measurement reads the state once for p0 and p1, then reads the kept half
-    and writes the whole state (2.5*N*s)."""
+    and writes the whole state (2.5*N*s).  Two gate kernels that the other
+    workloads' circuits do not reach are timed here too: the phase gate
+    (N*s: half the state read and written) and, on a reference-compatible
+    circuit, the out-of-place controlled gate (2*N*s)."""
     import ctypes
     import statistics
     import numpy as np
@@ -433,25 +436,41 @@ def bench_observe(args):
                          None),
         "probability": (lambda: ok(hip.qcs_hip_probability(st, 12345, ctypes.byref(d))),
                         2 * s, 2 * s, None),
+        # the phase gate P(0.3) on qubit 7 (k_1q_phase: reads and writes only
+        # the bit-7 = 1 half)
+        "phase_gate_q7": (lambda: ok(hip.qcs_hip_apply_1q(st, 7, pm)), N * s, N * s, None),
     }
+    pm = (ctypes.c_double * 8)(1.0, 0.0, 0.0, 0.0, 0.0, 0.0, math.cos(0.3), math.sin(0.3))
     out = {}
-    for name, (fn, nbytes, moved, setup) in ops.items():
-        times = []
-        for i in range(args.warmup + args.steps):
-            if setup:
-                setup()
-            hip.qcs_hip_device_sync()
-            t0 = time.perf_counter()
-            fn()
-            ok(hip.qcs_hip_sync(st))
-            if i >= args.warmup:
-                times.append(time.perf_counter() - t0)
-        t = statistics.median(times)
-        out[name] = {"ms": t * 1e3, "algorithmic_bytes": nbytes,
-                     "gbs": nbytes / t / 1e9, "hbm_frac": nbytes / t / 1e9 / HBM_PEAK_GBS,
-                     "moved_bytes": moved, "moved_gbs": moved / t / 1e9}
-        print(f"bench: observe {name} {t * 1e3:.3f} ms", file=sys.stderr, flush=True)
+
+    def time_ops(state, table):
+        for name, (fn, nbytes, moved, setup) in table.items():
+            times = []
+            for i in range(args.warmup + args.steps):
+                if setup:
+                    setup()
+                hip.qcs_hip_device_sync()
+                t0 = time.perf_counter()
+                fn()
+                ok(hip.qcs_hip_sync(state))
+                if i >= args.warmup:
+                    times.append(time.perf_counter() - t0)
+            t = statistics.median(times)
+            out[name] = {"ms": t * 1e3, "algorithmic_bytes": nbytes,
+                         "gbs": nbytes / t / 1e9, "hbm_frac": nbytes / t / 1e9 / HBM_PEAK_GBS,
+                         "moved_bytes": moved, "moved_gbs": moved / t / 1e9}
+            print(f"bench: observe {name} {t * 1e3:.3f} ms", file=sys.stderr, flush=True)
+
+    time_ops(st, ops)
     c.close()
+    # the reference-compatible controlled gate (out of place over the whole
+    # state, then a buffer swap: 2*N*s), on a compat circuit (two buffers)
+    cc = qcs_amd.Circuit(n, qcs_amd.REFERENCE_COMPAT)
+    cst = cc.device_state
+    xm = (ctypes.c_double * 8)(0.0, 0.0, 1.0, 0.0, 1.0, 0.0, 0.0, 0.0)
+    time_ops(cst, {"cnot_compat_3_9": (lambda: ok(hip.qcs_hip_apply_cu(cst, 3, 9, xm)),
+                                       2 * N * s, 2 * N * s, None)})
+    cc.close()
     print(json.dumps({
         "metric": f"observation kernels on a {n}-qubit complex128 state (GB/s of algorithmic bytes)",
         "value": out["norm_sq"]["gbs"], "unit": "GB/s",

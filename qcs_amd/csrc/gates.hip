@@ -221,29 +221,26 @@ static bool short_runs(int b, size_t amp_bytes) { return (size_t(1) << b) * amp_
 
 // Compat controlled gate, out of place over all N (src/q_gates.c:277-292 as
 // written): out[i] = m00*v[i] + m01*v[i|t] where bit c = 1 and bit t = 0,
-// out[i] = v[i] everywhere else (iteration i|t overwrites the partner).
+// out[i] = v[i] everywhere else (iteration i|t overwrites the partner).  One
+// work-item per (bit t = 0, bit t = 1) pair, so each amplitude is read and
+// written once.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_cu_compat(
-    const typename Amp<T>::type *in, typename Amp<T>::type *out, uint64_t n,
-    uint64_t cb, uint64_t tb, Mat2<T> m) {
+    const typename Amp<T>::type *in, typename Amp<T>::type *out, uint64_t npairs,
+    uint64_t cb, int t, Mat2<T> m) {
   typedef typename Amp<T>::type A;
-  const uint64_t nvb = ceil_div(n, uint64_t(kThreads) * kUnroll);
+  const uint64_t tb = uint64_t(1) << t;
+  const uint64_t nvb = ceil_div(npairs, uint64_t(kThreads));
   for (uint64_t vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
-  const uint64_t base = vb * (kThreads * kUnroll) + threadIdx.x;
-#pragma unroll
-  for (int u = 0; u < kUnroll; ++u) {
-    const uint64_t i = base + uint64_t(u) * kThreads;
-    if (i >= n) continue;
-    const A v = in[i];
-    if ((i & cb) && !(i & tb)) {
-      const A w = in[i | tb];
-      A o;
-      row2(m.v[0], m.v[1], m.v[2], m.v[3], v.x, v.y, w.x, w.y, o.x, o.y);
-      out[i] = o;
-    } else {
-      out[i] = v;
-    }
-  }
+    const uint64_t p = xcd_order(vb, nvb) * kThreads + threadIdx.x;
+    if (p >= npairs) continue;
+    const uint64_t i0 = insert0(p, t);
+    const A v0 = ld_stream(in + i0), v1 = ld_stream(in + (i0 | tb));
+    A o = v0;
+    if (i0 & cb)
+      row2(m.v[0], m.v[1], m.v[2], m.v[3], v0.x, v0.y, v1.x, v1.y, o.x, o.y);
+    st_stream(out + i0, o);
+    st_stream(out + (i0 | tb), v1);
   }
 }
 
@@ -351,8 +348,8 @@ static int do_cu(qcs_hip_state *s, int c, int t, const double md[8]) {
   if (s->flags & QCS_HIP_COMPAT) {
     A *out = static_cast<A *>(s->scratch);
     timing_begin(s, QCS_HIP_K_CU, 2.0 * N * sb, 0.0, &tl);
-    hipLaunchKernelGGL(k_cu_compat<T>, dim3(grid_for(s->size)), dim3(kThreads),
-                       0, s->stream, cur, out, s->size, cb, tb, m);
+    hipLaunchKernelGGL(k_cu_compat<T>, dim3(grid_for(s->size / 2)), dim3(kThreads),
+                       0, s->stream, cur, out, s->size / 2, cb, t, m);
     s->scratch = s->vec;
     s->vec = out;
   } else {
