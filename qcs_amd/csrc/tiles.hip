@@ -197,7 +197,8 @@ __device__ __forceinline__ BlockScalars load_scalars(const TileBlock *__restrict
 // current block's matrix-core chain runs.
 struct BlockRegs {
   f4v a[4];
-  int lp;
+  int lp;   // lane table entry l32
+  int lp2;  // lane table entry l32 ^ 16 (the 16-column form of k <= 3 blocks)
 };
 
 __device__ __forceinline__ void load_regs(BlockRegs &r, int blk) {
@@ -205,6 +206,7 @@ __device__ __forceinline__ void load_regs(BlockRegs &r, int blk) {
 #pragma unroll
   for (int v = 0; v < 4; ++v) r.a[v] = lds_load<f4v>(blk + (v * 64 + lane) * 16);
   r.lp = lds_load<int>(blk + kTileAFp32 + 4 * l32);
+  r.lp2 = lds_load<int>(blk + kTileAFp32 + 4 * (l32 ^ 16));
 }
 
 // What every block kernel needs of its block: uniform row offsets and the
@@ -299,6 +301,78 @@ __device__ __forceinline__ void tp_block(const TileBlock *__restrict__ bl,
   }
 }
 
+// One block of k <= 3 qubits in FP32 on v_mfma_f32_16x16x4f32.  The 32x32
+// form pads a block's 2^(k+1) real output rows to 32, wasting half the
+// matrix-core work of a k = 3 block and three quarters of a k = 2 one.  Here
+// a wave's 32-column group runs as two 16-column halves (lane bit 4 of the
+// column picks the half; lc0 and lc1 are the lane's column in each), and
+// lane l = 16q + c supplies A[l & 15][q] and B[q][c] with K-index q meaning
+// input row 2sp + (q & 1), part q >> 1, for instruction sp = 0 .. R/2 - 1.
+// The lane receives rows 4q + v of the 16x16 result, laid out as output row
+// (q & 1) | (v >> 1) << 1 | (q >> 1) << 2, part v & 1 (the host stages A in
+// that row order).  So each half-wave reads, and writes, one plane at two
+// rows that differ in the block's first target bit: it is conflict-free when
+// that bit's bank bit is none of the four low lane bits' (the host picks the
+// lanes so).  The next block's values are read as in tp_block.
+template <int K, int NTH, int TILE>
+__device__ __forceinline__ void tp_block16(const TileBlock *__restrict__ bl,
+                                           const BlockScalars &d, const BlockRegs &r,
+                                           const TileBlock *__restrict__ nbl, int nblk,
+                                           int wave, BlockScalars &nd, BlockRegs &nr) {
+  constexpr int R = 1 << K, NI = R / 2, NW = NTH / 64, GROUPS = (TILE >> K) / 32 / NW;
+  static_assert(K >= 1 && K <= 3, "the 16-column form takes k = 1..3");
+  static_assert(GROUPS >= 1 && GROUPS <= 16, "a wave takes 1..16 column groups");
+  const int lane = threadIdx.x & 63, q = lane >> 4;
+  float wa[NI];
+  int ro2[NI];
+#pragma unroll
+  for (int sp = 0; sp < NI; ++sp) {
+    wa[sp] = r.a[0][sp];
+    ro2[sp] = d.ro(2 * sp);
+  }
+  const int lc0 = (lane & 16) ? r.lp2 : r.lp, lc1 = (lane & 16) ? r.lp : r.lp2;
+  const int rq = (q & 1) ? d.rb[0] : 0;
+  const int bq = rq ^ ((q >> 1) ? kTilePlane : 0);  // B: row 2sp + (q & 1), part q >> 1
+  const int oq = rq ^ ((q >> 1) ? d.rb[2] : 0);     // result: rows (q & 1) | (q >> 1) << 2
+  if (GROUPS > 1) {
+    nd = load_scalars(nbl, wave);
+    load_regs(nr, nblk);
+  }
+#pragma unroll 1
+  for (int g = 0; g < GROUPS; ++g) {
+    const int up = GROUPS == 1 ? d.up : d.up ^ bl->jpart[g];
+    const int b0 = lc0 ^ bq ^ up, b1 = lc1 ^ bq ^ up;
+    float bv0[NI], bv1[NI];
+#pragma unroll
+    for (int sp = 0; sp < NI; ++sp) {
+      bv0[sp] = lds_load<float>(b0 ^ ro2[sp]);
+      bv1[sp] = lds_load<float>(b1 ^ ro2[sp]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    f4v acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int sp = 0; sp < NI; ++sp) {
+      acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(wa[sp], bv0[sp], acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(wa[sp], bv1[sp], acc1, 0, 0, 0);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    if (GROUPS == 1) {
+      nd = load_scalars(nbl, wave);
+      load_regs(nr, nblk);
+    }
+    const int o0 = lc0 ^ oq ^ up, o1 = lc1 ^ oq ^ up;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = (q & 1) | ((v >> 1) << 1) | ((q >> 1) << 2);
+      if (row < R) {
+        const int ov = ((v >> 1) ? d.rb[1] : 0) ^ ((v & 1) ? kTilePlane : 0);
+        lds_store<float>(o0 ^ ov, acc0[v]);
+        lds_store<float>(o1 ^ ov, acc1[v]);
+      }
+    }
+  }
+}
+
 // One FP32 block of the pass (index b), with the next block's values read
 // into (nd, nr); the block after the last is the pass's first (next tile).
 template <int NTH, int TILE>
@@ -309,9 +383,9 @@ __device__ __forceinline__ void tp_step(const TileBlock *__restrict__ blocks, in
   const TileBlock *bl = blocks + b, *nbl = blocks + bn;
   const int nblk = ops + bn * kTileStrideFp32;
   switch (d.k) {
-    case 1: tp_block<1, NTH, TILE>(bl, d, r, nbl, nblk, wave, nd, nr); break;
-    case 2: tp_block<2, NTH, TILE>(bl, d, r, nbl, nblk, wave, nd, nr); break;
-    case 3: tp_block<3, NTH, TILE>(bl, d, r, nbl, nblk, wave, nd, nr); break;
+    case 1: tp_block16<1, NTH, TILE>(bl, d, r, nbl, nblk, wave, nd, nr); break;
+    case 2: tp_block16<2, NTH, TILE>(bl, d, r, nbl, nblk, wave, nd, nr); break;
+    case 3: tp_block16<3, NTH, TILE>(bl, d, r, nbl, nblk, wave, nd, nr); break;
     default: tp_block<4, NTH, TILE>(bl, d, r, nbl, nblk, wave, nd, nr); break;
   }
   // consecutive blocks whose waves own the same amplitudes (same wave bits)
@@ -419,9 +493,16 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
     const int b = idx / (64 * 16), ln = (idx / 16) % 64, s = idx % 16;
     const TileBlock &bl = blocks[ps.first_block + b];
     const int R = 1 << bl.k, RR = 2 * R, row = ln & 31, hh = ln >> 5;
-    const float w = (row < RR && s < R)
-                        ? wbuf[size_t(bl.wslot) * kTileWFloats + row * RR + 2 * s + hh]
-                        : 0.0f;
+    const float *W = wbuf + size_t(bl.wslot) * kTileWFloats;
+    float w = (row < RR && s < R) ? W[row * RR + 2 * s + hh] : 0.0f;
+    if (!SPLIT && bl.k <= 3) {
+      // the 16-column form (tp_block16): instruction s, A[i][qq] with i =
+      // 4 qi + vi holding output row (qi & 1) | (vi >> 1) << 1 | (qi >> 1) << 2,
+      // part vi & 1, and qq meaning input row 2s + (qq & 1), part qq >> 1
+      const int i = ln & 15, qq = ln >> 4, qi = i >> 2, vi = i & 3;
+      const int ro = (qi & 1) | ((vi >> 1) << 1) | ((qi >> 1) << 2), si = 2 * s + (qq & 1);
+      w = (ro < R && si < R) ? W[(2 * ro + (vi & 1)) * RR + 2 * si + (qq >> 1)] : 0.0f;
+    }
     const int blk = OPS + b * STRIDE;
     if (!SPLIT) {
       // [s / 4][lane] float4, element s % 4
@@ -659,6 +740,37 @@ static int pick_lanes(int avail, int lanes[5]) {
   return 1 << (5 - distinct);
 }
 
+// Lane bits of a k <= 3 block in the 16-column form (tp_block16): lane bits
+// 0..3 on distinct bank bits other than tb0, the bank bit of the block's
+// first target (a half-wave reads 16 columns at two rows that differ in that
+// target), lane bit 4 (the half a column falls in) on tb0 where it can, the
+// rest the lowest positions left.  Returns the half-wave's conflict factor,
+// 2^(5 - rank of the bank bits of lane bits 0..3 and tb0).
+static int pick_lanes16(int avail, int tb0, int lanes[5]) {
+  int n = 0, used = 0;
+  for (int c = 0; c < 5 && n < 4; ++c) {
+    if (c == tb0) continue;
+    for (int p = 0; p < kTileBitsFixed; ++p)
+      if (((avail >> p) & 1) && bank_bit(p) == c) {
+        lanes[n++] = p;
+        used |= 1 << p;
+        break;
+      }
+  }
+  const int distinct = n + 1;
+  for (int p = 0; p < kTileBitsFixed && n < 4; ++p)
+    if (((avail >> p) & 1) && !((used >> p) & 1)) {
+      lanes[n++] = p;
+      used |= 1 << p;
+    }
+  lanes[4] = -1;
+  for (int p = 0; p < kTileBitsFixed && lanes[4] < 0; ++p)
+    if (((avail >> p) & 1) && !((used >> p) & 1) && bank_bit(p) == tb0) lanes[4] = p;
+  for (int p = 0; p < kTileBitsFixed && lanes[4] < 0; ++p)
+    if (((avail >> p) & 1) && !((used >> p) & 1)) lanes[4] = p;
+  return 1 << (5 - distinct);
+}
+
 static int target_mask(const qcs_hip_pass &ps, const qcs_hip_block &blk) {
   int m = 0;
   for (int q = 0; q < blk.k; ++q) {
@@ -680,9 +792,9 @@ static int scatter_off(int v, const int *pos, int npos) {
 // The block's descriptor for the given wave bits (a mask of `nwb` tile
 // positions, none of them a target): row offsets, lane, wave and group
 // tables.  Lane bits are conflict-free where the remaining positions allow,
-// else the lowest ones.
+// else the lowest ones; form16: the block runs in the 16-column form.
 static void make_tile_block(const qcs_hip_pass &ps, const qcs_hip_block &blk, int wslot,
-                            int wmask, int nwb, bool sync, TileBlock *tb) {
+                            int wmask, int nwb, bool sync, bool form16, TileBlock *tb) {
   std::memset(tb, 0, sizeof(TileBlock));
   const int k = blk.k, R = 1 << k;
   tb->k = k;
@@ -696,7 +808,10 @@ static void make_tile_block(const qcs_hip_pass &ps, const qcs_hip_block &blk, in
   for (int p = 0; p < kTileBitsFixed; ++p)
     if ((wmask >> p) & 1) wpos[nw++] = p;
   int avail = ((1 << kTileBitsFixed) - 1) & ~tmask & ~wmask;
-  (void)pick_lanes(avail, lanes);
+  if (form16)
+    (void)pick_lanes16(avail, bank_bit(tpos[0]), lanes);
+  else
+    (void)pick_lanes(avail, lanes);
   for (int i = 0; i < 5; ++i) avail &= ~(1 << lanes[i]);
   int nj = 0;
   for (int p = 0; p < kTileBitsFixed; ++p)
@@ -712,23 +827,29 @@ static void make_tile_block(const qcs_hip_pass &ps, const qcs_hip_block &blk, in
 // grows while the union of its targets leaves nwb tile bits free.  The wave
 // bits of a run are the choice with the fewest bank conflicts over its
 // blocks (a shared choice can leave some of them 2-way conflicted: on this
-// kernel a barrier costs more than that).
+// kernel a barrier costs more than that).  With FP32 math, blocks of k <= 3
+// run in the 16-column form, whose conflicts also depend on the first target.
 static void layout_pass(const qcs_hip_pass &ps, const qcs_hip_block *blocks, int nwb,
-                        TileBlock *tb) {
-  // conflict factor of every set of free positions, and the candidate wave
-  // bits of each width, computed once per process
+                        bool split, TileBlock *tb) {
+  // conflict factor of every set of free positions (and, for the 16-column
+  // form, bank bit of the first target), and the candidate wave bits of each
+  // width, computed once per process
   constexpr int kSets = 1 << kTileBitsFixed;
   static std::once_flag once;
-  static unsigned char factor[kSets];
+  static unsigned char factor[kSets], factor16[5][kSets];
   static std::vector<int> masks[5];
   std::call_once(once, [] {
     int lanes[5];
-    for (int a = 0; a < kSets; ++a)
-      factor[a] = __builtin_popcount(unsigned(a)) >= 5 ? pick_lanes(a, lanes) : 0;
+    for (int a = 0; a < kSets; ++a) {
+      const bool enough = __builtin_popcount(unsigned(a)) >= 5;
+      factor[a] = enough ? pick_lanes(a, lanes) : 0;
+      for (int c = 0; c < 5; ++c) factor16[c][a] = enough ? pick_lanes16(a, c, lanes) : 0;
+    }
     for (int m = 0; m < kSets; ++m)
       if (__builtin_popcount(unsigned(m)) <= 4) masks[__builtin_popcount(unsigned(m))].push_back(m);
   });
   const int end = ps.first_block + ps.num_blocks;
+  auto form16 = [&](int i) { return !split && blocks[i].k <= 3; };
   std::vector<int> tm(size_t(ps.num_blocks));
   for (int i = 0; i < ps.num_blocks; ++i) tm[i] = target_mask(ps, blocks[ps.first_block + i]);
   for (int b = ps.first_block; b < end;) {
@@ -742,14 +863,17 @@ static void layout_pass(const qcs_hip_pass &ps, const qcs_hip_block *blocks, int
     for (int m : masks[nwb]) {
       if (m & u) continue;
       int cost = 0;
-      for (int i = b; i < e; ++i) cost += factor[(kSets - 1) & ~tm[i - ps.first_block] & ~m];
+      for (int i = b; i < e; ++i) {
+        const int t = tm[i - ps.first_block], a = (kSets - 1) & ~t & ~m;
+        cost += form16(i) ? factor16[bank_bit(__builtin_ctz(unsigned(t)))][a] : factor[a];
+      }
       if (best < 0 || cost < best_cost) {
         best = m;
         best_cost = cost;
       }
     }
     for (int i = b; i < e; ++i)
-      make_tile_block(ps, blocks[i], i, best, nwb, i == e - 1, &tb[i]);
+      make_tile_block(ps, blocks[i], i, best, nwb, i == e - 1, form16(i), &tb[i]);
     b = e;
   }
 }
@@ -775,7 +899,7 @@ int tiles_apply(qcs_hip_state *s, const qcs_hip_pass *passes, int np,
     tp[p].nblocks = passes[p].num_blocks;
     tp[p].first_block = passes[p].first_block;
     for (int j = 0; j < passes[p].tile_bits; ++j) tp[p].sbits[j] = passes[p].bits[j];
-    layout_pass(passes[p], blocks, nwb, tb);
+    layout_pass(passes[p], blocks, nwb, s->tile_math == QCS_HIP_TILE_MATH_SPLIT_BF16, tb);
   }
   QCS_HIP_TRY(hipMemcpyAsync(s->tpass_dev, tp, pbytes, hipMemcpyHostToDevice,
                              s->stream));

@@ -136,8 +136,12 @@ def test_fused_random_circuits_vs_oracle_and_numpy(qlib, n):
 
 @pytest.mark.parametrize("n,threads", [(16, 512), (17, 1024), (20, 512), (20, 1024)])
 def test_tile_programs_equal_single_block_passes(qlib, n, threads):
-    """Tile programs apply the same composed blocks, in the same order, with
-    the same MFMA accumulation as one-pass-per-block: bit-identical."""
+    """Tile programs apply the same composed blocks, in the same order, as
+    one pass per block.  Blocks of four qubits use the same MFMA and
+    accumulation order in both kernels; smaller ones run on the 16x16x4
+    instruction in tile programs (no padding to 32 rows), which rounds
+    differently, so the states agree to FP32 rounding: a block applied out of
+    order, to the wrong bits or with a wrong operand would differ by O(1)."""
     from test_fusion_plan import pass_plan, to_gates  # noqa: F401
     hip, _ = qlib
     print(f"[fused] tile programs n={n} threads={threads}", flush=True)
@@ -159,7 +163,7 @@ def test_tile_programs_equal_single_block_passes(qlib, n, threads):
         rc = hip.qcs_hip_apply_fused(b.device_state, B, len(blocks), G, len(ordered))
         assert rc == 0, hip.qcs_hip_last_error()
         got, want = a.amplitudes(), b.amplitudes()
-    assert np.array_equal(got, want)
+    assert np.linalg.norm(got - want) <= 1e-5
     ref = HP.simulate(n, ops, psi.astype(np.complex64).astype(complex))
     assert 1 - abs(np.vdot(ref, got)) ** 2 / np.vdot(got, got).real <= 1e-6
 
