@@ -151,6 +151,12 @@ __device__ __forceinline__ void lds_store(int addr, T v) {
   *reinterpret_cast<lds_t *>(static_cast<size_t>(static_cast<unsigned>(addr))) = v;
 }
 
+// wait for every outstanding LDS access and scalar load of the wave
+__device__ __forceinline__ void lgkm_wait_all() {
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0); vmcnt and expcnt not waited for
+  __builtin_amdgcn_sched_barrier(0);
+}
+
 // x0, x1 -> packed bf16 pairs hi (round to nearest even) and lo (the rounded
 // remainder): x = hi + lo within 2^-16 |x|
 __device__ __forceinline__ void split_pair(float x0, float x1, unsigned &hi,
@@ -273,27 +279,28 @@ __device__ __forceinline__ void tp_block(const TileBlock *__restrict__ bl,
 #pragma unroll
   for (int s = 0; s < R; ++s) wa[s] = r.a[s >> 2][s & 3];
   const BlockGeom<K> geo(d, r.lp);
-  if (GROUPS > 1) {
-    nd = load_scalars(nbl, wave);
-    load_regs(nr, nblk);
-  }
 #pragma unroll 1
   for (int g = 0; g < GROUPS; ++g) {
     const int up = geo.template group_part<GROUPS>(bl, g);
     const int cbh = geo.lph ^ up;
+    // Nothing else may be outstanding on the LDS/scalar counter while the
+    // B reads return (scalar loads return out of order, so the compiler
+    // would wait for all of them before the first MFMA); the reads go out
+    // in the order the chain uses them, so each MFMA waits only for its own.
+    lgkm_wait_all();
     float bv[R];
 #pragma unroll
-    for (int s = 0; s < R; ++s) bv[s] = lds_load<float>(cbh ^ geo.ro[s]);
-    // all reads first, one wait, then the chain: interleaved, each MFMA
-    // would wait out an LDS round trip for its operand
-    __builtin_amdgcn_sched_barrier(0);
+    for (int s = 0; s < R; ++s) {
+      bv[s] = lds_load<float>(cbh ^ geo.ro[s]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
     floatx16 acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f,
                     0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int s = 0; s < R; ++s)
       acc = __builtin_amdgcn_mfma_f32_32x32x2f32(wa[s], bv[s], acc, 0, 0, 0);
     __builtin_amdgcn_sched_barrier(0);
-    if (GROUPS == 1) {
+    if (g == 0) {
       nd = load_scalars(nbl, wave);
       load_regs(nr, nblk);
     }
