@@ -215,6 +215,54 @@ __global__ __launch_bounds__(kThreads) void k_pair_sweep(typename Amp<T>::type *
   }
 }
 
+// One-qubit and controlled gates whose target bit t is so low that a pair's
+// members share a 128-byte line (2^t * s < 128: t <= 2 for complex128, <= 3
+// for complex64).  Taking the pairs along t, each load or store instruction
+// would cover every line of its span in part, and the other part with the
+// next instruction; measured at 30 qubits, complex128 1-qubit gates on
+// qubits 0..2 took 6.5-6.75 ms against 5.9-6.0 ms on qubits 3..27.  Here
+// lane l of a wave holds amplitude base + l, so every access instruction
+// moves 64 consecutive amplitudes, and the partner's value comes from lane
+// l ^ 2^t (t < 6) by a cross-lane read.  The result is the one k_1q_dense
+// (or k_pair_sweep) computes, with the same operand order:
+//   KIND 0: m applied where the control bits cm are all set (cm = 0: always);
+//   KIND 1: the members exchanged where they are (CNOT).
+// Every amplitude is read and written once (2 N s).
+template <int KIND, typename T>
+__global__ __launch_bounds__(kThreads) void k_lane_pairs(typename Amp<T>::type *a,
+                                                         uint64_t n, int t, uint64_t cm,
+                                                         Mat2<T> m) {
+  typedef typename Amp<T>::type A;
+  const uint64_t nvb = ceil_div(n, uint64_t(kThreads));
+  const int src = int((threadIdx.x & 63u) ^ (1u << t));
+  for (uint64_t vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
+    const uint64_t e = xcd_order(vb, nvb) * kThreads + threadIdx.x;
+    // every lane takes part in the exchange; a lane past the end (states of
+    // fewer than 8 qubits) is never a partner of one inside, since 2^t < n
+    A v;
+    v.x = 0;
+    v.y = 0;
+    if (e < n) v = ld_stream(a + e);
+    A w;
+    w.x = __shfl(v.x, src);
+    w.y = __shfl(v.y, src);
+    if (e >= n) continue;
+    A o = v;
+    if ((e & cm) == cm) {
+      if (KIND == 1)
+        o = w;
+      else if ((e >> t) & 1)
+        row2(m.v[4], m.v[5], m.v[6], m.v[7], w.x, w.y, v.x, v.y, o.x, o.y);
+      else
+        row2(m.v[0], m.v[1], m.v[2], m.v[3], v.x, v.y, w.x, w.y, o.x, o.y);
+    }
+    st_stream(a + e, o);
+  }
+}
+
+// true when a pair along bit t lies within one 128-byte line
+static bool lane_pairs(int t, size_t amp_bytes) { return (size_t(1) << t) * amp_bytes < 128; }
+
 // true when the runs of a half-state sweep with lowest fixed bit `b` are
 // shorter than a 64-byte line
 static bool short_runs(int b, size_t amp_bytes) { return (size_t(1) << b) * amp_bytes < 64; }
@@ -321,6 +369,10 @@ static int do_1q(qcs_hip_state *s, int t, const double md[8]) {
     timing_begin(s, QCS_HIP_K_1Q, N * sb, 3.0 * N, &tl);
     hipLaunchKernelGGL(k_1q_phase<T>, dim3(grid_for(nhalf)), dim3(kThreads), 0,
                        s->stream, cur, nhalf, t, m.v[6], m.v[7]);
+  } else if (!compat && lane_pairs(t, sizeof(A))) {
+    timing_begin(s, QCS_HIP_K_1Q, 2.0 * N * sb, 14.0 * N, &tl);
+    hipLaunchKernelGGL((k_lane_pairs<0, T>), dim3(grid_for(s->size)), dim3(kThreads), 0,
+                       s->stream, cur, s->size, t, uint64_t(0), m);
   } else {
     A *out = compat ? static_cast<A *>(s->scratch) : cur;
     const uint64_t npairs = s->size / 2;
@@ -363,7 +415,13 @@ static int do_cu(qcs_hip_state *s, int c, int t, const double md[8]) {
       timing_begin(s, QCS_HIP_K_CU, (ph ? 0.5 : 1.0) * N * sb, 0.0, &tl);
       const uint64_t np = s->size / 2;
       const dim3 gp(grid_for(np));
-      if (x)
+      if (!ph && lane_pairs(t, sizeof(A))) {
+        const dim3 ge(grid_for(s->size));
+        if (x)
+          hipLaunchKernelGGL((k_lane_pairs<1, T>), ge, b, 0, s->stream, cur, s->size, t, cb, m);
+        else
+          hipLaunchKernelGGL((k_lane_pairs<0, T>), ge, b, 0, s->stream, cur, s->size, t, cb, m);
+      } else if (x)
         hipLaunchKernelGGL((k_pair_sweep<1, false, T>), gp, b, 0, s->stream, cur, np, t, cb, m);
       else if (!ph)
         hipLaunchKernelGGL((k_pair_sweep<0, false, T>), gp, b, 0, s->stream, cur, np, t, cb, m);
