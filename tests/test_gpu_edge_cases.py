@@ -1,8 +1,9 @@
 """Edge cases on the device: one-qubit circuits in every form, indices out
 of range, invalid measurements and zero-shot calls (which draw no uniform,
 as the reference returns before rand(): src/qcs.c:245-248, 581-582),
-invalid gates on a fused circuit, amplitude ranges past the state, and the
-batch API's argument checks on a real batch."""
+invalid gates on a fused circuit, amplitude ranges past the state, the
+batch API's argument checks on a real batch, and the device gate entry
+points on every bit and bit pair."""
 import ctypes
 import math
 
@@ -141,3 +142,39 @@ def test_argmax_ties_go_to_the_first_index(qlib):
             amps[N - 2] = 0.5                      # smaller
             c.set_amplitudes(amps)
             assert c.find_most_likely_state() == i
+
+
+@pytest.mark.parametrize("options,tol", [(qcs_amd.PLAIN_FP64, 1e-14), (qcs_amd.PLAIN_FP32, 2e-6)])
+def test_device_gates_on_every_bit_pair(qlib, options, tol):
+    """qcs_hip_apply_1q / qcs_hip_apply_cu with a general matrix (and X) on
+    every target, and every (control, target) pair, of an 8-qubit state:
+    each launch path (a pair within one 128-byte line, one lane per
+    amplitude; low control bits, whole-line sweeps; the half-state kernels)
+    against the numpy statement of the reference's loops
+    (src/q_gates.c:146-148, 277-292)."""
+    import helpers as HP
+    hip, _ = qlib
+    n = 8
+    rng = np.random.default_rng(5)
+    u, _ = np.linalg.qr(rng.standard_normal((2, 2)) + 1j * rng.standard_normal((2, 2)))
+    x = np.array([[0, 1], [1, 0]], complex)
+    psi = rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)
+    psi /= np.linalg.norm(psi)
+    with qcs_amd.Circuit(n, options) as c:
+        st = c.device_state
+        for t in range(n):
+            for m in (u, x):
+                c.set_amplitudes(psi)
+                mm = (ctypes.c_double * 8)(*HP.m8(m))
+                assert hip.qcs_hip_apply_1q(st, t, mm) == 0
+                got, want = c.amplitudes(), HP.apply_1q(psi, n, t, m)
+                assert np.max(np.abs(got - want)) < tol, ("1q", t)
+            for ctl in range(n):
+                if ctl == t:
+                    continue
+                for m in (u, x):
+                    c.set_amplitudes(psi)
+                    mm = (ctypes.c_double * 8)(*HP.m8(m))
+                    assert hip.qcs_hip_apply_cu(st, ctl, t, mm) == 0
+                    got, want = c.amplitudes(), HP.apply_cu(psi, n, ctl, t, m)
+                    assert np.max(np.abs(got - want)) < tol, ("cu", ctl, t)
