@@ -430,9 +430,14 @@ int qcs_hip_qubit_probs(qcs_hip_state *s, int qubit, double *p0, double *p1) {
 int qcs_hip_measure(qcs_hip_state *s, int qubit, double u, int *outcome) {
   int rc;
   if ((rc = check_state(s)) || (rc = check_qubit(s, qubit, "measure"))) return rc;
+  // p0 over the bit = 0 half (src/qcs.c:253-259); p1, the norm the kept
+  // half has after zeroing, only when the outcome is 1.  Where the halves'
+  // runs are shorter than a line, one pass over the whole state gives both.
   double p0 = 0.0, p1 = 0.0;
-  if ((rc = qubit_probs(s, qubit, &p0, &p1))) return rc;
+  const bool halves = half_prob_pays(s, qubit);
+  if ((rc = halves ? half_prob(s, qubit, 0, &p0) : qubit_probs(s, qubit, &p0, &p1))) return rc;
   const int o = (u <= p0) ? 0 : 1;  // src/qcs.c:263
+  if (o == 1 && halves && (rc = half_prob(s, qubit, 1, &p1))) return rc;
   // after zeroing, q_state_normalize sums only the kept half
   const double ns = o == 0 ? p0 : p1;
   const int do_scale = (ns > 1e-12 && ns != 1.0) ? 1 : 0;

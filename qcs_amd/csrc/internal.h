@@ -124,6 +124,10 @@ int reduce_norm(qcs_hip_state *s, double *out_host);  // host result
 int normalize_device(qcs_hip_state *s);               // no host sync
 int scale_state(qcs_hip_state *s, double factor);
 int qubit_probs(qcs_hip_state *s, int q, double *p0, double *p1);
+// probability of bit q == bit, reading only that half of the state; worth it
+// where the half's runs of 2^q amplitudes are at least a 128-byte line
+bool half_prob_pays(const qcs_hip_state *s, int q);
+int half_prob(qcs_hip_state *s, int q, int bit, double *p);
 int collapse(qcs_hip_state *s, int q, int outcome, double factor, int do_scale);
 int argmax(qcs_hip_state *s, long *idx);
 int sample(qcs_hip_state *s, long shots, const double *u, long *out);

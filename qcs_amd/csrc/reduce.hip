@@ -59,6 +59,37 @@ __global__ __launch_bounds__(kThreads) void k_norm_partial(
   }
 }
 
+// Sum |a|^2 over the half of the state whose bit q is set (bq = 2^q) or
+// clear (bq = 0): half-index j is amplitude insert0(j, q) | bq, so the half
+// is read in runs of 2^q amplitudes (the caller uses it where a run is at
+// least a 128-byte line).  Same loop shape as k_norm_partial; slot 0 only.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_half_norm_partial(
+    const typename Amp<T>::type *a, uint64_t nh, int q, uint64_t bq, double *partial) {
+  __shared__ double lds[kThreads / 64];
+  double s0 = 0.0;
+  const uint64_t nvb = ceil_div(nh, uint64_t(kThreads)), G = gridDim.x;
+  auto at = [&](uint64_t vb) {
+    return insert0(xcd_order(vb, nvb) * kThreads + threadIdx.x, q) | bq;
+  };
+  auto inside = [&](uint64_t vb) { return xcd_order(vb, nvb) * kThreads + threadIdx.x < nh; };
+  uint64_t vb = blockIdx.x;
+  for (; vb + 3 * G < nvb; vb += 4 * G) {
+    // as in k_norm_partial, only vb + 3G can be the last, partial block
+    const typename Amp<T>::type v0 = ld_stream(a + at(vb)), v1 = ld_stream(a + at(vb + G)),
+                                v2 = ld_stream(a + at(vb + 2 * G));
+    const double p3 = inside(vb + 3 * G) ? norm2(ld_stream(a + at(vb + 3 * G))) : 0.0;
+    s0 += norm2(v0);
+    s0 += norm2(v1);
+    s0 += norm2(v2);
+    s0 += p3;
+  }
+  for (; vb < nvb; vb += G)
+    if (inside(vb)) s0 += norm2(ld_stream(a + at(vb)));
+  s0 = block_sum(s0, lds);
+  if (threadIdx.x == 0) partial[blockIdx.x * kRedSlots + 0] = s0;
+}
+
 // Fold `nb` partials of `nslots` slots: slots in max_mask take the maximum,
 // the others the sum.  Results go to out[0..nslots).
 __global__ __launch_bounds__(kThreads) void k_final(const double *partial,
@@ -653,6 +684,35 @@ int qubit_probs(qcs_hip_state *s, int q, double *p0, double *p1) {
   if ((rc = fetch_results(s, 2))) return rc;
   *p0 = s->host[0];
   *p1 = s->host[1];
+  return 0;
+}
+
+template <typename T>
+static int half_partials(qcs_hip_state *s, int q, int bit, unsigned nb) {
+  typedef typename Amp<T>::type A;
+  const uint64_t nh = s->size / 2;
+  TimedLaunch tl;
+  timing_begin(s, QCS_HIP_K_REDUCE, double(nh) * sizeof(A), 3.0 * double(nh), &tl);
+  hipLaunchKernelGGL(k_half_norm_partial<T>, dim3(nb), dim3(kThreads), 0, s->stream,
+                     static_cast<const A *>(s->vec), nh, q,
+                     bit ? (uint64_t(1) << q) : uint64_t(0), s->red);
+  int rc = launch_check("k_half_norm_partial");
+  if (rc) return rc;
+  return timing_end(s, &tl);
+}
+
+bool half_prob_pays(const qcs_hip_state *s, int q) {
+  return (size_t(1) << q) * size_t(s->amp_bytes) >= 128;
+}
+
+int half_prob(qcs_hip_state *s, int q, int bit, double *p) {
+  const unsigned nb = red_blocks(s->size / 2);
+  int rc = s->precision == QCS_HIP_C128 ? half_partials<double>(s, q, bit, nb)
+                                        : half_partials<float>(s, q, bit, nb);
+  if (rc) return rc;
+  if ((rc = fold(s, nb, 1, 0u))) return rc;
+  if ((rc = fetch_results(s, 1))) return rc;
+  *p = s->host[0];
   return 0;
 }
 
