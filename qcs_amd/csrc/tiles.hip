@@ -320,7 +320,8 @@ __device__ __forceinline__ void tp_block(const TileBlock *__restrict__ bl,
 // that row order).  So each half-wave reads, and writes, one plane at two
 // rows that differ in the block's first target bit: it is conflict-free when
 // that bit's bank bit is none of the four low lane bits' (the host picks the
-// lanes so).  The next block's values are read as in tp_block.
+// lanes so).  The next block's values are read before the first column
+// group when there are several (after the chain when there is one).
 template <int K, int NTH, int TILE>
 __device__ __forceinline__ void tp_block16(const TileBlock *__restrict__ bl,
                                            const BlockScalars &d, const BlockRegs &r,
