@@ -35,10 +35,12 @@ __global__ __launch_bounds__(kThreads) void k_norm_partial(
   uint64_t vb = blockIdx.x;
   for (; vb + 3 * G < nvb; vb += 4 * G) {
     const uint64_t i0 = at(vb), i1 = at(vb + G), i2 = at(vb + 2 * G), i3 = at(vb + 3 * G);
-    // a virtual block other than the last is full; the last is vb + 3G at most
-    const typename Amp<T>::type v0 = ld_stream(a + i0), v1 = ld_stream(a + i1),
-                                v2 = ld_stream(a + i2);
-    const double p0 = norm2(v0), p1 = norm2(v1), p2 = norm2(v2);
+    // xcd_order can put the grid's one partial virtual block at any of the
+    // four slots, so each load is guarded (a skipped term adds 0.0: the
+    // association of the sum is unchanged)
+    const double p0 = i0 < n ? norm2(ld_stream(a + i0)) : 0.0;
+    const double p1 = i1 < n ? norm2(ld_stream(a + i1)) : 0.0;
+    const double p2 = i2 < n ? norm2(ld_stream(a + i2)) : 0.0;
     const double p3 = i3 < n ? norm2(ld_stream(a + i3)) : 0.0;
     if (i0 & qb) s1 += p0; else s0 += p0;
     if (i1 & qb) s1 += p1; else s0 += p1;
@@ -75,13 +77,14 @@ __global__ __launch_bounds__(kThreads) void k_half_norm_partial(
   auto inside = [&](uint64_t vb) { return xcd_order(vb, nvb) * kThreads + threadIdx.x < nh; };
   uint64_t vb = blockIdx.x;
   for (; vb + 3 * G < nvb; vb += 4 * G) {
-    // as in k_norm_partial, only vb + 3G can be the last, partial block
-    const typename Amp<T>::type v0 = ld_stream(a + at(vb)), v1 = ld_stream(a + at(vb + G)),
-                                v2 = ld_stream(a + at(vb + 2 * G));
+    // each of the four may be the partial virtual block (see k_norm_partial)
+    const double p0 = inside(vb) ? norm2(ld_stream(a + at(vb))) : 0.0;
+    const double p1 = inside(vb + G) ? norm2(ld_stream(a + at(vb + G))) : 0.0;
+    const double p2 = inside(vb + 2 * G) ? norm2(ld_stream(a + at(vb + 2 * G))) : 0.0;
     const double p3 = inside(vb + 3 * G) ? norm2(ld_stream(a + at(vb + 3 * G))) : 0.0;
-    s0 += norm2(v0);
-    s0 += norm2(v1);
-    s0 += norm2(v2);
+    s0 += p0;
+    s0 += p1;
+    s0 += p2;
     s0 += p3;
   }
   for (; vb < nvb; vb += G)
@@ -206,12 +209,15 @@ __global__ __launch_bounds__(kThreads) void k_argmax_partial(
   uint64_t vb = blockIdx.x;
   for (; vb + 3 * G < nvb; vb += 4 * G) {
     const uint64_t i0 = at(vb), i1 = at(vb + G), i2 = at(vb + 2 * G), i3 = at(vb + 3 * G);
-    const double p0 = norm2(ld_stream(a + i0)), p1 = norm2(ld_stream(a + i1)),
-                 p2 = norm2(ld_stream(a + i2));
-    take(p0, i0);
-    take(p1, i1);
-    take(p2, i2);
-    if (i3 < n) take(norm2(ld_stream(a + i3)), i3);
+    // any of the four may be the partial virtual block (see k_norm_partial)
+    const double p0 = i0 < n ? norm2(ld_stream(a + i0)) : -1.0;
+    const double p1 = i1 < n ? norm2(ld_stream(a + i1)) : -1.0;
+    const double p2 = i2 < n ? norm2(ld_stream(a + i2)) : -1.0;
+    const double p3 = i3 < n ? norm2(ld_stream(a + i3)) : -1.0;
+    if (i0 < n) take(p0, i0);
+    if (i1 < n) take(p1, i1);
+    if (i2 < n) take(p2, i2);
+    if (i3 < n) take(p3, i3);
   }
   for (; vb < nvb; vb += G) {
     const uint64_t i = at(vb);
