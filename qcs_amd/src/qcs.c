@@ -825,8 +825,9 @@ void qc_ghz_state(t_q_circuit *circuit) {
 /* utilities                                                                */
 /* ---------------------------------------------------------------------- */
 
-int qc_get_num_qubits(t_q_circuit *circuit) { return circuit->num_qubits; }
-int qc_get_num_gates(t_q_circuit *circuit) { return circuit->num_gates; }
+/* src/qcs.c getters; a NULL circuit reads as 0 (the reference dereferences it) */
+int qc_get_num_qubits(t_q_circuit *circuit) { return circuit ? circuit->num_qubits : 0; }
+int qc_get_num_gates(t_q_circuit *circuit) { return circuit ? circuit->num_gates : 0; }
 
 /* src/qcs.c:650-698: drop adjacent identical H/X/Y/Z pairs on one qubit and
  * adjacent identical CNOT pairs from the history, restarting the scan after

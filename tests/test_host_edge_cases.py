@@ -45,6 +45,8 @@ def test_null_circuit_is_a_no_op(q):
     assert q.qc_norm_sq(None) == 0.0
     assert q.qc_get_device_state(None) is None
     assert q.qc_get_amplitudes(None, 0, 1, dbuf) < 0
+    assert q.qc_get_num_qubits(None) == 0
+    assert q.qc_get_num_gates(None) == 0
     assert q.qc_set_amplitudes(None, 0, 1, dbuf) < 0
 
 
