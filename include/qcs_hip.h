@@ -137,6 +137,22 @@ int qcs_hip_argmax(qcs_hip_state *s, long *index);
  * materialised. */
 int qcs_hip_sample(qcs_hip_state *s, long shots, const double *u,
                    long *index_out);
+/* qcs_hip_sample_noalloc: the same samples as qcs_hip_sample for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  It
+ * calls no hipMalloc, hipFree or hipHostMalloc: `u_dev` (shots doubles) and
+ * `index_out_dev` (shots longs) are device-accessible buffers of the caller,
+ * and the FP64 chunk sums live in one file-scope device array per GPU of
+ * 2 * 2^(QCS_HIP_NOALLOC_MAX_QUBITS - 12) doubles (4 MiB), held until the
+ * process ends.  States of more than QCS_HIP_NOALLOC_MAX_QUBITS qubits are
+ * rejected with QCS_HIP_EINVAL.  The work is queued on the state's stream and
+ * the call returns without waiting: qcs_hip_sync(s) before reading
+ * index_out_dev.  Every call on one GPU shares that array, so calls whose
+ * device work overlaps (states on different streams, or several host
+ * threads) race; the caller orders them, e.g. qcs_hip_sync between calls. */
+#define QCS_HIP_NOALLOC_MAX_QUBITS 30
+int qcs_hip_sample_noalloc(qcs_hip_state *s, long shots, const double *u_dev,
+                           long *index_out_dev);
 
 /* ---- fused precision form ----------------------------------------------
  * A gate for the fusion pipeline (textbook semantics). */

@@ -98,6 +98,7 @@ _HIP_PROTOS = {
     "qcs_hip_probability": (c_int, [c_void_p, c_long, P(c_double)]),
     "qcs_hip_argmax": (c_int, [c_void_p, P(c_long)]),
     "qcs_hip_sample": (c_int, [c_void_p, c_long, P(c_double), P(c_long)]),
+    "qcs_hip_sample_noalloc": (c_int, [c_void_p, c_long, c_void_p, c_void_p]),
     "qcs_hip_apply_fused": (c_int, [c_void_p, P(HipBlock), c_int, P(HipGate), c_int]),
     "qcs_hip_apply_unitary": (c_int, [c_void_p, c_int, P(c_int), P(c_double)]),
     "qcs_hip_compare": (c_int, [c_void_p, c_void_p, P(c_double), P(c_double)]),

@@ -477,6 +477,32 @@ int qcs_hip_sample(qcs_hip_state *s, long shots, const double *u,
   return sample(s, shots, u, index_out);
 }
 
+// a buffer the device can address (device, managed or pinned host memory);
+// an unregistered host pointer would fault the kernel instead of failing
+static int check_device_buffer(const void *p, const char *what) {
+  if (!p) return fail(QCS_HIP_EINVAL, "%s: null buffer", what);
+  hipPointerAttribute_t attr;
+  if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+    (void)hipGetLastError();  // not a launch error of a later call
+    return fail(QCS_HIP_EINVAL, "%s: %p is not device-accessible memory", what, p);
+  }
+  if (attr.type != hipMemoryTypeDevice && attr.type != hipMemoryTypeManaged &&
+      !(attr.type == hipMemoryTypeHost && attr.devicePointer == p))
+    return fail(QCS_HIP_EINVAL, "%s: %p is not device-accessible memory", what, p);
+  return 0;
+}
+
+int qcs_hip_sample_noalloc(qcs_hip_state *s, long shots, const double *u_dev,
+                           long *index_out_dev) {
+  int rc;
+  if ((rc = check_state(s))) return rc;
+  if (shots <= 0) return 0;
+  if ((rc = check_device_buffer(u_dev, "sample_noalloc uniforms")) ||
+      (rc = check_device_buffer(index_out_dev, "sample_noalloc indices")))
+    return rc;
+  return sample_noalloc(s, shots, u_dev, index_out_dev);
+}
+
 int qcs_hip_apply_fused(qcs_hip_state *s, const qcs_hip_block *blocks,
                         int num_blocks, const qcs_hip_gate *gates,
                         int num_gates) {

@@ -131,6 +131,9 @@ int half_prob(qcs_hip_state *s, int q, int bit, double *p);
 int collapse(qcs_hip_state *s, int q, int outcome, double factor, int do_scale);
 int argmax(qcs_hip_state *s, long *idx);
 int sample(qcs_hip_state *s, long shots, const double *u, long *out);
+// qcs_hip_sample's kernels on caller-supplied device buffers, with the chunk
+// sums in the file-scope scratch array of reduce.hip (no allocation)
+int sample_noalloc(qcs_hip_state *s, long shots, const double *u_dev, long *idx_dev);
 int grover_textbook(qcs_hip_state *s, long marked, long iters);
 int phase_flip_compat(qcs_hip_state *s, long index);
 int diffusion_compat(qcs_hip_state *s);

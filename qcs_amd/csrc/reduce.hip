@@ -7,6 +7,7 @@
 // order, so results are bit-reproducible run to run.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include "device.h"
 #include "internal.h"
@@ -830,6 +831,80 @@ static int sample_impl(qcs_hip_state *s, long shots, const double *u,
 int sample(qcs_hip_state *s, long shots, const double *u, long *out) {
   return s->precision == QCS_HIP_C128 ? sample_impl<double>(s, shots, u, out)
                                       : sample_impl<float>(s, shots, u, out);
+}
+
+// ---- sampling with no allocation ---------------------------------------------
+//
+// qcs_hip_sample_noalloc keeps the chunk totals (tot) and inclusive prefixes
+// (inc) in one file-scope device array per GPU instead of buffers grown per
+// state.  Its device address is looked up once per device with that device
+// current; host code passes that address to the kernels, never the array's
+// name (in host code the name is a host-side stand-in).  Every entry
+// [0, nchunks) of both halves is written by k_chunk_tot / k_chunk_scan before
+// any kernel reads it, so nothing needs clearing between calls.
+constexpr uint64_t kNoAllocChunks = uint64_t(1) << (QCS_HIP_NOALLOC_MAX_QUBITS - 12);
+static_assert(kChunk == 4096, "kNoAllocChunks assumes 4096-amplitude chunks");
+static __device__ __align__(256) double scratch[2 * kNoAllocChunks];
+
+constexpr int kMaxDevices = 64;
+static std::once_flag g_scratch_once[kMaxDevices];
+static double *g_scratch_ptr[kMaxDevices];
+static hipError_t g_scratch_err[kMaxDevices];
+
+static int noalloc_scratch(int device, double **out) {
+  if (device < 0 || device >= kMaxDevices)
+    return fail(QCS_HIP_EUNSUP, "sample_noalloc: device %d beyond %d", device, kMaxDevices);
+  std::call_once(g_scratch_once[device], [device] {
+    int cur = 0;
+    hipError_t e = hipGetDevice(&cur);
+    if (e == hipSuccess && cur != device) e = hipSetDevice(device);
+    void *p = nullptr;
+    if (e == hipSuccess) e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) (void)hipSetDevice(cur);
+    g_scratch_err[device] = e;
+    g_scratch_ptr[device] = e == hipSuccess ? static_cast<double *>(p) : nullptr;
+  });
+  if (g_scratch_err[device] != hipSuccess)
+    return check_hip(g_scratch_err[device], "hipGetSymbolAddress(scratch)");
+  *out = g_scratch_ptr[device];
+  return 0;
+}
+
+template <typename T>
+static int sample_noalloc_impl(qcs_hip_state *s, double *scr, long shots,
+                               const double *u_dev, long *idx_dev) {
+  typedef typename Amp<T>::type A;
+  const uint64_t len = s->size < uint64_t(kChunk) ? s->size : uint64_t(kChunk);
+  const uint64_t nchunks = s->size / len;
+  double *tot = scr, *inc = scr + kNoAllocChunks;
+  const A *a = static_cast<const A *>(s->vec);
+  hipLaunchKernelGGL(k_chunk_tot<T>, dim3(unsigned(ceil_div(nchunks, kThreads / 64))),
+                     dim3(kThreads), 0, s->stream, a, nchunks, len, tot);
+  int rc = launch_check("k_chunk_tot");
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_chunk_scan, dim3(1), dim3(1024), 0, s->stream, tot, inc, nchunks);
+  if ((rc = launch_check("k_chunk_scan"))) return rc;
+  for (uint64_t s0 = 0; s0 < uint64_t(shots); s0 += uint64_t(1) << 24) {
+    const uint64_t ns = std::min<uint64_t>(uint64_t(shots) - s0, uint64_t(1) << 24);
+    hipLaunchKernelGGL(k_sample<T>, dim3(unsigned(ceil_div(ns, kThreads / 64))),
+                       dim3(kThreads), 0, s->stream, a, nchunks, len, tot, inc,
+                       u_dev + s0, ns, idx_dev + s0);
+    if ((rc = launch_check("k_sample"))) return rc;
+  }
+  return 0;
+}
+
+int sample_noalloc(qcs_hip_state *s, long shots, const double *u_dev, long *idx_dev) {
+  if (s->n > QCS_HIP_NOALLOC_MAX_QUBITS)
+    return fail(QCS_HIP_EINVAL,
+                "sample_noalloc: %d qubits exceed the scratch array's %d (%llu chunk sums)",
+                s->n, QCS_HIP_NOALLOC_MAX_QUBITS, (unsigned long long)kNoAllocChunks);
+  double *scr = nullptr;
+  int rc = noalloc_scratch(s->device, &scr);
+  if (rc) return rc;
+  return s->precision == QCS_HIP_C128
+             ? sample_noalloc_impl<double>(s, scr, shots, u_dev, idx_dev)
+             : sample_noalloc_impl<float>(s, scr, shots, u_dev, idx_dev);
 }
 
 template <typename T>
