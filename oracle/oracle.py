@@ -56,6 +56,9 @@ def lib():
             "oq_qft": (None, [c_void_p]),
             "oq_grover": (None, [c_void_p, c_long, c_long]),
             "oq_apply_1q_omp": (None, [c_void_p, c_int, PD, c_int]),
+            "oq_apply_cu_omp": (None, [c_void_p, c_int, c_int, PD, c_int]),
+            "oq_run_omp": (None, [c_void_p, c_int, PI, PI, PI, PD, c_int]),
+            "oq_set_threads": (None, [c_int]),
         }
         for name, (res, args) in protos.items():
             fn = getattr(L, name)
@@ -108,6 +111,12 @@ class OracleState:
         op, tq, cq, par = ops.arrays()
         self.L.oq_run(self.s, len(op), op.ctypes.data_as(PI), tq.ctypes.data_as(PI),
                       cq.ctypes.data_as(PI), _dp(par))
+
+    def run_omp(self, ops, racy=False):
+        """ops with the OpenMP loops (race-fixed unless racy)"""
+        op, tq, cq, par = ops.arrays()
+        self.L.oq_run_omp(self.s, len(op), op.ctypes.data_as(PI), tq.ctypes.data_as(PI),
+                          cq.ctypes.data_as(PI), _dp(par), 1 if racy else 0)
 
     def apply_1q(self, t, m):
         m = np.ascontiguousarray(m, np.float64)

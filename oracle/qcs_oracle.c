@@ -17,13 +17,21 @@
  * re = ar*br - ai*bi, im = ar*bi + ai*br); built with gcc -std=c89, which does
  * not contract into FMA, exactly like the reference's own build.
  *
- * Also holds the OpenMP restatement of the reference's CPU-parallel 1q loop
- * (src/q_gates.c:40-68) for the CPU baseline: one variant keeps the shared
- * inner index of the original (a data race), one makes it private.
+ * Also holds the OpenMP restatement of the reference's CPU-parallel loops
+ * for the CPU baseline: the 1q loop (src/q_gates.c:40-68), where one variant
+ * keeps the shared inner index of the original (a data race) and one makes it
+ * private, and the 2q loop (src/q_gates.c:174-208), where the original's
+ * (c=1, t=1) iteration rewrites the partner that the (c=1, t=0) iteration
+ * wrote (a write-write race between threads) and the fixed variants write
+ * each entry from one iteration only.  The race-fixed variants equal the
+ * sequential functions above bit for bit.
  */
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #include "qcs_oracle.h"
 
@@ -445,4 +453,98 @@ void oq_apply_1q_omp(oq_state *s, int t, const double m[8], int racy) {
     }
   }
   swap_buffers(s);
+}
+
+/* ---- OpenMP restatement of the CPU-parallel 2q loop (src/q_gates.c:174-208)
+ * mode 0: race-fixed, textbook: the (c=1, t=0) iteration writes both entries
+ *         of its pair and the (c=1, t=1) iteration writes nothing;
+ * mode 1: race-fixed, as written: the result the sequential loop leaves
+ *         (out[i1] = v1, its later else branch), the (c=1, t=0) iteration
+ *         writing only i0;
+ * mode 2: as written, racy: both iterations write i1, possibly on two
+ *         threads. */
+void oq_apply_cu_omp(oq_state *s, int c, int t, const double m[8], int mode) {
+  long cb, tb, i;
+  ocx g0, g1, g2, g3;
+  if (c < 0 || t < 0 || c >= s->n || t >= s->n || c == t) return;
+  cb = 1L << c;
+  tb = 1L << t;
+  g0 = mat(m, 0);
+  g1 = mat(m, 1);
+  g2 = mat(m, 2);
+  g3 = mat(m, 3);
+  memcpy(s->scr, s->vec, (size_t)s->size * sizeof(ocx));
+  if (mode == 2) {
+#pragma omp parallel for
+    for (i = 0; i < s->size; i++) {
+      if ((i & cb) != 0 && (i & tb) == 0) {
+        ocx v0 = s->vec[i], v1 = s->vec[i | tb];
+        s->scr[i] = cx_add(cx_mul(g0, v0), cx_mul(g1, v1));
+        s->scr[i | tb] = cx_add(cx_mul(g2, v0), cx_mul(g3, v1));
+      } else {
+        s->scr[i] = s->vec[i];
+      }
+    }
+  } else if (mode == 1) {
+#pragma omp parallel for
+    for (i = 0; i < s->size; i++) {
+      if ((i & cb) != 0 && (i & tb) == 0) {
+        ocx v0 = s->vec[i], v1 = s->vec[i | tb];
+        s->scr[i] = cx_add(cx_mul(g0, v0), cx_mul(g1, v1));
+      } else {
+        s->scr[i] = s->vec[i];
+      }
+    }
+  } else {
+#pragma omp parallel for
+    for (i = 0; i < s->size; i++) {
+      if ((i & cb) != 0 && (i & tb) == 0) {
+        ocx v0 = s->vec[i], v1 = s->vec[i | tb];
+        s->scr[i] = cx_add(cx_mul(g0, v0), cx_mul(g1, v1));
+        s->scr[i | tb] = cx_add(cx_mul(g2, v0), cx_mul(g3, v1));
+      } else if ((i & cb) == 0) {
+        s->scr[i] = s->vec[i];
+      }
+    }
+  }
+  swap_buffers(s);
+}
+
+/* oq_run with the OpenMP loops: racy = 1 runs both loops as the reference's
+ * OpenMP build writes them; racy = 0 runs the race-fixed loops in the state's
+ * semantics (textbook or as written).  SWAP stays sequential (no reference
+ * counterpart). */
+void oq_run_omp(oq_state *s, int ng, const int *op, const int *tq, const int *cq,
+                const double *par, int racy) {
+  int i;
+  double m[8];
+  const int mode = racy ? 2 : (s->compat ? 1 : 0);
+  for (i = 0; i < ng; i++) {
+    switch (op[i]) {
+      case OQ_CNOT:
+        oq_gate(OQ_X, 0.0, m);
+        oq_apply_cu_omp(s, cq[i], tq[i], m, mode);
+        break;
+      case OQ_CPHASE:
+        oq_gate(OQ_P, par[i], m);
+        oq_apply_cu_omp(s, cq[i], tq[i], m, mode);
+        break;
+      case OQ_SWAP:
+        oq_apply_swap(s, tq[i], cq[i]);
+        break;
+      default:
+        oq_gate(op[i], par[i], m);
+        oq_apply_1q_omp(s, tq[i], m, racy);
+        break;
+    }
+  }
+}
+
+/* threads of the OpenMP loops (the runtime this library was linked with) */
+void oq_set_threads(int threads) {
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#else
+  (void)threads;
+#endif
 }

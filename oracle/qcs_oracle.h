@@ -41,5 +41,9 @@ void oq_run(oq_state *s, int ng, const int *op, const int *tq, const int *cq,
 void oq_qft(oq_state *s);
 void oq_grover(oq_state *s, long marked, long iterations);
 void oq_apply_1q_omp(oq_state *s, int t, const double m[8], int racy);
+void oq_apply_cu_omp(oq_state *s, int c, int t, const double m[8], int mode);
+void oq_run_omp(oq_state *s, int ng, const int *op, const int *tq, const int *cq,
+                const double *par, int racy);
+void oq_set_threads(int threads);
 
 #endif

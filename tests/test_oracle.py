@@ -255,3 +255,20 @@ def test_openmp_restatement_matches_sequential():
         b.L.oq_apply_1q_omp(b.s, int(t), m.ctypes.data_as(__import__("ctypes").POINTER(
             __import__("ctypes").c_double)), 0)
     assert np.array_equal(a.vector(), b.vector())
+
+
+@pytest.mark.parametrize("compat", [False, True])
+def test_openmp_race_fixed_circuits_match_sequential(compat):
+    # the race-fixed 1q and 2q OpenMP loops on 4 threads give the sequential
+    # oracle's states bit for bit, in both semantics (textbook and the
+    # reference's as-written controlled gate)
+    n = 13
+    ops = C.random_circuit(n, 60, 17, kinds=[C.H, C.RX, C.RY, C.RZ, C.P, C.CNOT, C.CPHASE])
+    a, b = OracleState(n, compat=compat), OracleState(n, compat=compat)
+    a.L.oq_set_threads(4)
+    a.run(ops)
+    b.run_omp(ops, racy=False)
+    assert np.array_equal(a.vector(), b.vector())
+    assert np.array_equal(a.scratch(), b.scratch())
+    a.close()
+    b.close()
