@@ -86,6 +86,19 @@ struct BCirc {
 };
 constexpr uint64_t kNoSlot = ~uint64_t(0);
 
+// One queue item: a circuit, or one of the `nparts` parts of a streamed
+// complex128 circuit that several work-groups share (part j takes tiles
+// j, j + nparts, ... of every pass; the parts meet between passes).
+struct BItem {
+  int circ;
+  int part;
+  int nparts;
+  int reserved;
+};
+// spins of a part's wait at a pass boundary before it gives up (each spin
+// sleeps ~1000 clocks: several seconds, far beyond any pass)
+constexpr unsigned kPartSpinLimit = 1u << 23;
+
 struct BPass {
   int first_op;
   int nops;
@@ -633,10 +646,41 @@ __device__ void sample_state(const P &prob, int n, double *tot, double *inc,
   __syncthreads();
 }
 
-// ---- one circuit per work-group ---------------------------------------------------
+// ---- one circuit per work-group, or one part of a shared circuit ------------------
+
+// Pass boundary of the work-groups that share a circuit.  Every thread's
+// tile stores are made visible device-wide (agent-scope release: across
+// the XCDs' L2s) before the work-group adds itself to the circuit's
+// counter; thread 0 then waits for `target` arrivals and acquires.  The wait
+// is bounded: on timeout, or when another part has timed out, the error flag
+// is set and every wave of the work-group returns false.
+__device__ bool parts_barrier(unsigned *cnt, unsigned target, unsigned *err) {
+  __shared__ int ok_s;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int ok = 1;
+    atomicAdd(cnt, 1u);
+    for (unsigned spins = 0;
+         __hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target;) {
+      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
+          ++spins > kPartSpinLimit) {
+        atomicOr(err, 1u);
+        ok = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(16);
+    }
+    __threadfence();
+    ok_s = ok;
+  }
+  __syncthreads();
+  return ok_s != 0;
+}
 
 template <int NTH>
-__device__ void run_c128(const BCirc c, const BPass *__restrict__ passes,
+__device__ void run_c128(const BCirc c, int part, int nparts, unsigned *sync,
+                         unsigned *err, const BPass *__restrict__ passes,
                          const BGate *__restrict__ gates,
                          const BGroup *__restrict__ groups, char *states,
                          char *lds, double *tot, double *inc, double *red,
@@ -651,7 +695,10 @@ __device__ void run_c128(const BCirc c, const BPass *__restrict__ passes,
   const uint64_t ntiles = uint64_t(1) << (n - tb);
   for (int p = 0; p < c.npass; ++p) {
     const BPass &ps = passes[c.first_pass + p];
-    for (uint64_t t = 0; t < ntiles; ++t) {
+    // a pass reads tiles the previous pass wrote, possibly in another part
+    if (nparts > 1 && p > 0 && !parts_barrier(sync, unsigned(p) * unsigned(nparts), err))
+      return;
+    for (uint64_t t = uint64_t(part); t < ntiles; t += uint64_t(nparts)) {
       const uint64_t base = tile_base(t, ps.sbits, tb);
       if (p == 0 || resident) {
         if (p == 0)
@@ -676,6 +723,9 @@ __device__ void run_c128(const BCirc c, const BPass *__restrict__ passes,
       }
     }
   }
+  // the last pass done by every part; part 0 samples the state
+  if (nparts > 1 && !parts_barrier(sync, unsigned(c.npass) * unsigned(nparts), err)) return;
+  if (part != 0) return;
   if (resident && keep && G) {
     for (int e = tid; e < tile; e += NTH) G[e] = L[dswz(e)];
     __syncthreads();
@@ -775,14 +825,20 @@ __device__ void run_c64(const BCirc c, const BPass *__restrict__ passes,
 
 // The persistent kernel.  `queue` starts at 0 (k_batch_reset); each
 // work-group takes circuits until the counter passes nc.
+// The persistent kernel.  `queue[0]` starts at 0 (k_batch_reset); each
+// work-group takes queue items until the counter passes nitems.  The parts
+// of a shared circuit are consecutive items, at most half the grid, and
+// every work-group is resident, so the parts a waiting part needs are
+// always taken by work-groups that make progress.  queue[1] is the error
+// flag of the part barriers; sync[c] counts circuit c's pass arrivals.
 template <int NTH>
 __global__ __launch_bounds__(NTH) void k_batch(
-    const BCirc *__restrict__ circs, const int *__restrict__ order, int nc,
+    const BCirc *__restrict__ circs, const BItem *__restrict__ items, int nitems,
     const BPass *__restrict__ passes, const BGate *__restrict__ gates,
     const BGroup *__restrict__ groups, const BBlock *__restrict__ blocks,
     const float *__restrict__ wbuf, char *states, const double *__restrict__ u,
-    long *__restrict__ out, uint64_t shots, unsigned *queue, int tile_bytes,
-    int nch_cap, int keep) {
+    long *__restrict__ out, uint64_t shots, unsigned *queue, unsigned *sync,
+    int tile_bytes, int nch_cap, int keep) {
   extern __shared__ double4 smem4[];
   char *lds = reinterpret_cast<char *>(smem4);
   double *tot = reinterpret_cast<double *>(lds + tile_bytes);
@@ -794,19 +850,24 @@ __global__ __launch_bounds__(NTH) void k_batch(
     __syncthreads();
     const int i = __builtin_amdgcn_readfirstlane(next);
     __syncthreads();
-    if (i >= nc) break;
-    const BCirc c = circs[order[i]];
+    if (i >= nitems) break;
+    const BItem it = items[i];
+    const BCirc c = circs[it.circ];
     if (c.prec == QCS_HIP_C128)
-      run_c128<NTH>(c, passes, gates, groups, states, lds, tot, inc, red, u,
-                    out, shots, keep);
+      run_c128<NTH>(c, it.part, it.nparts, sync + it.circ, queue + 1, passes, gates,
+                    groups, states, lds, tot, inc, red, u, out, shots, keep);
     else
       run_c64<NTH>(c, passes, blocks, wbuf, states, lds, tot, inc, red, u,
                    out, shots, keep);
   }
 }
 
-__global__ void k_batch_reset(unsigned *queue) {
-  if (threadIdx.x == 0) queue[0] = 0u;
+// queue counter, error flag and the circuits' pass counters to zero (a
+// kernel rather than a memset, so a captured graph replays it reliably)
+__global__ void k_batch_reset(unsigned *queue, unsigned *sync, int nc) {
+  if (blockIdx.x == 0 && threadIdx.x < 2) queue[threadIdx.x] = 0u;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += gridDim.x * blockDim.x)
+    sync[i] = 0u;
 }
 
 // ---- host side ---------------------------------------------------------------------
@@ -814,7 +875,7 @@ __global__ void k_batch_reset(unsigned *queue) {
 // byte offsets of the regions of the descriptor block (device and pinned
 // staging alike)
 struct BatchLayout {
-  size_t circ, order, pass, gate, group, block, hgate, hblock, u, total;
+  size_t circ, order, sync, pass, gate, group, block, hgate, hblock, u, total;
 };
 
 }  // namespace qcs
@@ -835,7 +896,8 @@ struct qcs_hip_batch {
   size_t states_cap = 0;
   long *d_out = nullptr;
   size_t out_cap = 0;
-  unsigned *d_queue = nullptr;
+  unsigned *d_queue = nullptr;  // [0] queue counter, [1] part-barrier error flag
+  unsigned *h_flag = nullptr;   // pinned copy of the error flag
   // pinned staging (same layout as d_desc) and output
   char *h_stage = nullptr;
   size_t stage_cap = 0;
@@ -851,7 +913,7 @@ struct qcs_hip_batch {
   // launch of the last successful run, repeated by qcs_hip_batch_rerun
   bool has_last = false;
   qcs::BatchLayout last_layout = {};
-  int last_nc = 0, last_tile_bytes = 0, last_nch_cap = 0, last_grid = 0;
+  int last_nc = 0, last_nitems = 0, last_tile_bytes = 0, last_nch_cap = 0, last_grid = 0;
   long last_shots = 0;
   // last run, for read_state and timing
   std::vector<int> last_n, last_prec;
@@ -1076,20 +1138,20 @@ static BBlock make_bblock(const qcs_hip_pass &ps, const qcs_hip_block &bl,
 }
 
 template <int NTH>
-static int launch_batch(qcs_hip_batch *b, const BatchLayout &L, int nc,
+static int launch_batch(qcs_hip_batch *b, const BatchLayout &L, int nitems,
                         uint64_t shots, int tile_bytes, int nch_cap, int grid) {
   char *d = static_cast<char *>(b->d_desc);
   hipLaunchKernelGGL(k_batch<NTH>, dim3(unsigned(grid)), dim3(NTH),
                      size_t(tile_bytes) + 2 * sizeof(double) * size_t(nch_cap),
                      b->stream, reinterpret_cast<const BCirc *>(d + L.circ),
-                     reinterpret_cast<const int *>(d + L.order), nc,
+                     reinterpret_cast<const BItem *>(d + L.order), nitems,
                      reinterpret_cast<const BPass *>(d + L.pass),
                      reinterpret_cast<const BGate *>(d + L.gate),
                      reinterpret_cast<const BGroup *>(d + L.group),
                      reinterpret_cast<const BBlock *>(d + L.block), b->d_w,
                      b->d_states, reinterpret_cast<const double *>(d + L.u),
-                     b->d_out, shots, b->d_queue, tile_bytes, nch_cap,
-                     (b->flags & QCS_HIP_BATCH_KEEP_STATES) ? 1 : 0);
+                     b->d_out, shots, b->d_queue, reinterpret_cast<unsigned *>(d + L.sync),
+                     tile_bytes, nch_cap, (b->flags & QCS_HIP_BATCH_KEEP_STATES) ? 1 : 0);
   return launch_check("k_batch");
 }
 
@@ -1110,11 +1172,11 @@ static int batch_occupancy(int nth, int lds_bytes, int *per_cu) {
   }
 }
 
-static int batch_launch(qcs_hip_batch *b, int nth, const BatchLayout &L, int nc,
+static int batch_launch(qcs_hip_batch *b, int nth, const BatchLayout &L, int nitems,
                         uint64_t shots, int tile_bytes, int nch_cap, int grid) {
   switch (nth) {
-    case 256: return launch_batch<256>(b, L, nc, shots, tile_bytes, nch_cap, grid);
-    default: return launch_batch<512>(b, L, nc, shots, tile_bytes, nch_cap, grid);
+    case 256: return launch_batch<256>(b, L, nitems, shots, tile_bytes, nch_cap, grid);
+    default: return launch_batch<512>(b, L, nitems, shots, tile_bytes, nch_cap, grid);
   }
 }
 
@@ -1139,15 +1201,19 @@ static int enqueue_run(qcs_hip_batch *b, const BatchLayout &L, int nc, int nb,
                            reinterpret_cast<const qcs_hip_block *>(d + L.hblock),
                            nb, b->d_w, nullptr)))
     return rc;
-  hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, b->stream, b->d_queue);
+  hipLaunchKernelGGL(k_batch_reset, dim3(unsigned(std::min<uint64_t>(ceil_div(nc, 256), 64))),
+                     dim3(256), 0, b->stream, b->d_queue,
+                     reinterpret_cast<unsigned *>(d + L.sync), nc);
   if ((rc = launch_check("k_batch_reset"))) return rc;
   if (timed) QCS_HIP_TRY(hipEventRecord(b->ev[1], b->stream));
-  if ((rc = batch_launch(b, b->nth, L, nc, shots, tile_bytes, nch_cap, grid)))
+  if ((rc = batch_launch(b, b->nth, L, b->last_nitems, shots, tile_bytes, nch_cap, grid)))
     return rc;
   if (timed) QCS_HIP_TRY(hipEventRecord(b->ev[2], b->stream));
   if (ubytes)
     QCS_HIP_TRY(hipMemcpyAsync(b->h_out, b->d_out, ubytes, hipMemcpyDeviceToHost,
                                b->stream));
+  QCS_HIP_TRY(hipMemcpyAsync(b->h_flag, b->d_queue + 1, sizeof(unsigned),
+                             hipMemcpyDeviceToHost, b->stream));
   return 0;
 }
 
@@ -1193,6 +1259,9 @@ static int submit(qcs_hip_batch *b, bool rerun, int nb, bool build_graph) {
   }
   QCS_HIP_TRY(hipEventRecord(b->ev[3], b->stream));
   QCS_HIP_TRY(hipStreamSynchronize(b->stream));
+  if (*b->h_flag != 0u)
+    return fail(QCS_HIP_EDEVICE, "batch: the work-groups sharing a circuit did not all "
+                                 "reach a pass boundary in time");
   float ms = 0.0f;
   QCS_HIP_TRY(hipEventElapsedTime(&ms, b->ev[0], b->ev[3]));
   b->total_ms = ms;
@@ -1311,25 +1380,6 @@ int batch_run(qcs_hip_batch *b, const qcs_hip_batch_circuit *cs, int nc,
     n_cgate += size_t(blocks[bi].num_gates);
   }
 
-  // layout of the descriptor block (device and pinned staging alike)
-  BatchLayout L;
-  size_t off = 0;
-  auto take = [&](size_t bytes) {
-    const size_t at = off;
-    off = align_up(off + (bytes ? bytes : 1));
-    return at;
-  };
-  L.circ = take(sizeof(BCirc) * size_t(nc));
-  L.order = take(sizeof(int) * size_t(nc));
-  L.pass = take(sizeof(BPass) * size_t(np));
-  L.gate = take(sizeof(BGate) * n_gate);
-  L.group = take(sizeof(BGroup) * n_group);
-  L.block = take(sizeof(BBlock) * n_block);
-  L.hgate = take(sizeof(qcs_hip_gate) * n_cgate);
-  L.hblock = take(sizeof(qcs_hip_block) * size_t(nb));
-  L.u = take(sizeof(double) * size_t(nc) * size_t(shots));
-  L.total = off;
-
   // state slots and per-circuit cost
   const bool keep = (b->flags & QCS_HIP_BATCH_KEEP_STATES) != 0;
   std::vector<uint64_t> slot(size_t(nc), kNoSlot);
@@ -1363,6 +1413,65 @@ int batch_run(qcs_hip_batch *b, const qcs_hip_batch_circuit *cs, int nc,
       1, (uint64_t(1) << max_n) / std::min<uint64_t>(uint64_t(1) << max_n, kChunk)));
   const int lds_bytes = max_tile_bytes + int(2 * sizeof(double)) * nch_cap;
 
+  // persistent grid: every work-group resident at once
+  int per_cu = 0;
+  if ((rc = batch_occupancy(b->nth, lds_bytes, &per_cu))) return rc;
+  if (per_cu < 1) return fail(QCS_HIP_EDEVICE, "batch kernel does not fit on a CU");
+  const int grid = std::min(nc, per_cu * b->num_cus);
+  // A streamed complex128 circuit that costs several times a work-group's
+  // share of the batch is shared by that many work-groups (at most 32, its
+  // tile count, and half the grid), so no single circuit sets the launch
+  // time.  QCS_BATCH_MAX_PARTS lowers the cap (1: no sharing).
+  std::vector<int> shares(size_t(nc), 1);
+  {
+    int cap = 32;
+    if (const char *e = std::getenv("QCS_BATCH_MAX_PARTS")) cap = std::max(1, std::atoi(e));
+    cap = std::min(cap, std::max(1, grid / 2));
+    double total = 0.0;
+    for (int i = 0; i < nc; ++i) total += cost[size_t(i)];
+    const double share = total / double(std::max(grid, 1));
+    for (int i = 0; i < nc; ++i) {
+      const qcs_hip_batch_circuit &c = cs[i];
+      if (c.precision != QCS_HIP_C128 || c.tile_bits >= c.num_qubits) continue;
+      const uint64_t ntiles = uint64_t(1) << (c.num_qubits - c.tile_bits);
+      const int w = int(std::min<double>({cost[size_t(i)] / share, double(cap), double(ntiles)}));
+      if (w >= 2) shares[size_t(i)] = w;
+    }
+  }
+  // queue items: circuits by cost, largest first; the parts of a shared
+  // circuit are consecutive
+  std::vector<BItem> items;
+  {
+    std::vector<int> idx(static_cast<size_t>(nc));
+    for (int i = 0; i < nc; ++i) idx[size_t(i)] = i;
+    std::stable_sort(idx.begin(), idx.end(),
+                     [&](int a, int c) { return cost[size_t(a)] > cost[size_t(c)]; });
+    items.reserve(size_t(nc));
+    for (int i : idx)
+      for (int j = 0; j < shares[size_t(i)]; ++j) items.push_back(BItem{i, j, shares[size_t(i)], 0});
+  }
+  const int nitems = int(items.size());
+
+  // layout of the descriptor block (device and pinned staging alike)
+  BatchLayout L;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = off;
+    off = align_up(off + (bytes ? bytes : 1));
+    return at;
+  };
+  L.circ = take(sizeof(BCirc) * size_t(nc));
+  L.order = take(sizeof(BItem) * size_t(nitems));
+  L.sync = take(sizeof(unsigned) * size_t(nc));
+  L.pass = take(sizeof(BPass) * size_t(np));
+  L.gate = take(sizeof(BGate) * n_gate);
+  L.group = take(sizeof(BGroup) * n_group);
+  L.block = take(sizeof(BBlock) * n_block);
+  L.hgate = take(sizeof(qcs_hip_gate) * n_cgate);
+  L.hblock = take(sizeof(qcs_hip_block) * size_t(nb));
+  L.u = take(sizeof(double) * size_t(nc) * size_t(shots));
+  L.total = off;
+
   const double tb3 = host_ms();
   // buffers
   if ((rc = grow_device(b, &b->d_desc, &b->desc_cap, L.total))) return rc;
@@ -1392,7 +1501,6 @@ int batch_run(qcs_hip_batch *b, const qcs_hip_batch_circuit *cs, int nc,
   // descriptors into the pinned staging block
   char *h = b->h_stage;
   BCirc *hc = reinterpret_cast<BCirc *>(h + L.circ);
-  int *ho = reinterpret_cast<int *>(h + L.order);
   for (int i = 0; i < nc; ++i) {
     const qcs_hip_batch_circuit &c = cs[i];
     BCirc &d = hc[i];
@@ -1447,22 +1555,10 @@ int batch_run(qcs_hip_batch *b, const qcs_hip_batch_circuit *cs, int nc,
   if (nb > 0) std::memcpy(h + L.hblock, cblk.data(), sizeof(qcs_hip_block) * size_t(nb));
   if (shots > 0)
     std::memcpy(h + L.u, u, sizeof(double) * size_t(nc) * size_t(shots));
-  {
-    std::vector<int> idx(static_cast<size_t>(nc));
-    for (int i = 0; i < nc; ++i) idx[size_t(i)] = i;
-    std::stable_sort(idx.begin(), idx.end(),
-                     [&](int a, int c) { return cost[size_t(a)] > cost[size_t(c)]; });
-    std::memcpy(ho, idx.data(), sizeof(int) * size_t(nc));
-  }
+  std::memcpy(h + L.order, items.data(), sizeof(BItem) * items.size());
 
   const double t2 = host_ms();
-  // persistent grid: every work-group resident at once
-  int per_cu = 0;
-  if ((rc = batch_occupancy(b->nth, lds_bytes, &per_cu))) return rc;
-  if (per_cu < 1) return fail(QCS_HIP_EDEVICE, "batch kernel does not fit on a CU");
-  const int grid = std::min(nc, per_cu * b->num_cus);
-
-  std::vector<long> key = {b->generation, nc, np, nb, long(n_cgate), long(n_gate),
+  std::vector<long> key = {b->generation, nc, nitems, np, nb, long(n_cgate), long(n_gate),
                            long(n_group), long(n_block), shots, b->nth,
                            max_tile_bytes, nch_cap, grid, long(L.total),
                            long(states_bytes)};
@@ -1470,6 +1566,7 @@ int batch_run(qcs_hip_batch *b, const qcs_hip_batch_circuit *cs, int nc,
   if (new_graph) drop_graph(b);
   b->last_layout = L;
   b->last_nc = nc;
+  b->last_nitems = nitems;
   b->last_shots = shots;
   b->last_tile_bytes = max_tile_bytes;
   b->last_nch_cap = nch_cap;
@@ -1545,6 +1642,10 @@ int qcs_hip_batch_create(qcs_hip_batch **out, int flags) {
     if ((e = hipEventCreate(&b->ev[i])) != hipSuccess) rc = check_hip(e, "hipEventCreate");
   if (!rc && (e = hipMalloc(reinterpret_cast<void **>(&b->d_queue), kBAlign)) != hipSuccess)
     rc = check_hip(e, "hipMalloc(queue)");
+  if (!rc && (e = hipHostMalloc(reinterpret_cast<void **>(&b->h_flag), kBAlign,
+                                hipHostMallocDefault)) != hipSuccess)
+    rc = check_hip(e, "hipHostMalloc(flag)");
+  if (!rc) *b->h_flag = 0u;
   if (rc) {
     qcs_hip_batch_destroy(b);
     return rc;
@@ -1564,6 +1665,7 @@ void qcs_hip_batch_destroy(qcs_hip_batch *b) {
   if (b->d_queue) (void)hipFree(b->d_queue);
   if (b->h_stage) (void)hipHostFree(b->h_stage);
   if (b->h_out) (void)hipHostFree(b->h_out);
+  if (b->h_flag) (void)hipHostFree(b->h_flag);
   for (int i = 0; i < 4; ++i)
     if (b->ev[i]) (void)hipEventDestroy(b->ev[i]);
   if (b->stream) (void)hipStreamDestroy(b->stream);

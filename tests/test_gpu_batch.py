@@ -262,3 +262,27 @@ def test_rerun_reuses_the_last_plan(qlib, cl, capfd, monkeypatch, graph):
         c.close()
     b.close()
     r.close()
+
+
+def test_shared_large_circuits_equal_standalone(qlib, cl, monkeypatch):
+    # streamed complex128 circuits of 18-20 qubits at the default tile bits,
+    # beside small ones: the large ones are shared by several work-groups
+    # (their tiles dealt out per pass, the parts meeting between passes), and
+    # must give what one work-group gives (QCS_BATCH_MAX_PARTS=1) and what a
+    # circuit of its own gives, amplitudes and samples bit for bit
+    specs = [(20, C.layered(20, 5, 61)), (19, C.layered(19, 7, 62)),
+             (18, make_ops(18, 63))] + [(n, make_ops(n, 70 + n)) for n in (3, 9, 12, 13, 14)]
+    monkeypatch.setenv("QCS_BATCH_MAX_PARTS", "1")
+    _, a1, i1 = batch_run(specs, qcs_amd.PLAIN_FP64, 300, cl, 17)
+    monkeypatch.delenv("QCS_BATCH_MAX_PARTS")
+    b, a2, i2 = batch_run(specs, qcs_amd.PLAIN_FP64, 300, cl, 17,
+                          batch_opts=qcs_amd.BATCH_KEEP_STATES | qcs_amd.BATCH_GRAPH)
+    cl.srand(17)
+    for i, (n, ops) in enumerate(specs):
+        with qcs_amd.Circuit(n) as c:
+            c.apply(ops)
+            own_idx = c.sample_indices(300)
+            own = c.amplitudes()
+        assert np.array_equal(a2[i], own) and np.array_equal(i2[i], own_idx), n
+        assert np.array_equal(a1[i], own) and np.array_equal(i1[i], own_idx), n
+    b.close()
