@@ -61,18 +61,23 @@ def parse():
                    help="tiles: several fused blocks per HBM pass; split: tiles whose "
                         "blocks run as split-bf16 products; blocks: one pass per block")
     p.add_argument("--tile-threads", type=int, default=1024)
+    p.add_argument("--k4-form", choices=["chain", "halves"], default=None,
+                   help="form of 4-qubit blocks in FP32 tile programs (1024 threads): one "
+                        "32x32 chain per wave, or two pipelined 16-column halves; default: "
+                        "the autotuner's choice (chain with --no-tune)")
     p.add_argument("--no-plain", action="store_true", help="skip the plain FP64 run")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--no-alt", action="store_true",
                    help="skip the timing of the other fused modes")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
-    p.add_argument("--workload", choices=["fused30", "batch", "qft", "observe", "config5"],
+    p.add_argument("--workload", choices=["fused30", "batch", "qft", "observe", "config5", "cpu"],
                    default="fused30",
                    help="fused30: the 30-qubit fused-vs-plain line (default); batch: "
                         "4096 independent 10-20 qubit circuits in one persistent launch; "
                         "qft: the 28-qubit QFT with swaps, plain complex128 and fused; "
                         "observe: the observation kernels on a 30-qubit complex128 state; "
-                        "config5: 33-qubit Grover (complex128) and the 34-qubit fused circuit")
+                        "config5: 33-qubit Grover (complex128) and the 34-qubit fused circuit; "
+                        "cpu: config 1 in full on every CPU build of the reference (no GPU)")
     p.add_argument("--batch-count", type=int, default=4096)
     p.add_argument("--qft-qubits", type=int, default=28,
                    help="qubits of the QFT workload (default 28)")
@@ -201,6 +206,138 @@ def cpu_baseline(seconds):
         "nproc": os.cpu_count(),
         **omp,
     }
+
+
+def _ref_leg(ref, n, ops, shots, budget_s, what, cores):
+    """One reference build on config 1: apply the gates one call each until
+    done or `budget_s` passed, then (when all ran) qc_run_shots after
+    srand(0).  Amp-updates/s = gates * 2^n / time (the GPU's definition)."""
+    import ctypes
+    import qcs_amd
+    c = ref.create(n)
+    done, t0 = 0, time.perf_counter()
+    while done < len(ops) and time.perf_counter() - t0 < budget_s:
+        ref.apply(c, _slice(ops, done))
+        done += 1
+    dt = time.perf_counter() - t0
+    leg = {"value": done * float(1 << n) / dt, "unit": "amplitude updates/s",
+           "cores": cores, "kind": "reference", "gates_per_s": done / dt,
+           "gates_run": done, "seconds": dt, "build": what}
+    if done == len(ops):
+        hist = (ctypes.c_int * (1 << n))()
+        qcs_amd.libc().srand(0)
+        t1 = time.perf_counter()
+        ref.L.qc_run_shots(c, shots, hist)
+        ts = time.perf_counter() - t1
+        leg["shots_s"] = ts
+        leg["shots_per_s"] = shots / ts
+        leg["shots_landed"] = int(sum(hist))
+        leg["sample"] = (f"config 1 in full: {len(ops)} gates in {dt:.1f} s, then {shots} "
+                         f"shots after srand(0) in {ts:.2f} s ({leg['shots_landed']} landed)")
+    else:
+        leg["sample"] = (f"first {done} of {len(ops)} gates of config 1 within {dt:.1f} s "
+                         "(the full circuit would take minutes)")
+    ref.L.qc_destroy(c)
+    return leg
+
+
+def _omp_threads(k):
+    """threads of every OpenMP loop in this process (one libgomp instance)"""
+    import ctypes
+    try:
+        ctypes.CDLL("libgomp.so.1").omp_set_num_threads(int(k))
+    except OSError:
+        pass
+
+
+def bench_cpu(args):
+    """Config 1 on the host cores (not a GPU line): the 20-qubit depth-10
+    layered circuit, seed 0, 810 gates, then 1024 shots after srand(0), with
+    every CPU build the reference's path has, each a leg of its own:
+      * the reference's sequential build at its own flags (-std=c89 -O3
+        -march=x86-64-v3 -mavx2 -mfma) and at -O2, 1 core, in full;
+      * the reference's OpenMP build (as written: the 1q loop's shared index
+        and the 2q loop's double write race) at 1 thread in full and at
+        nproc threads on a bounded sample (its scaling is negative);
+      * the oracle's race-fixed OpenMP restatement of both loops, textbook and
+        as-written semantics, at 1 thread and at nproc, in full.
+    The reference builds are oracle/_ref (compiled from its sources); without
+    them only the oracle legs run."""
+    from qcs_amd import circuits as C
+    from oracle.oracle import OracleState
+    import qcs_amd
+    import numpy as np
+    n, depth, shots = 20, 10, 1024
+    ops = C.layered(n, depth, 0)
+    nproc = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    legs = {}
+    for lib, what, budget in (("libqcsref.so", "sequential, -O3 -march=x86-64-v3 -mavx2 -mfma", 120),
+                              ("libqcsref_O2.so", "sequential, -O2", 120)):
+        try:
+            from oracle.oracle import RefLib
+            ref = RefLib(lib)
+        except (FileNotFoundError, OSError):
+            continue
+        key = "reference_" + ("O3" if "O2" not in lib else "O2")
+        legs[key] = _ref_leg(ref, n, ops, shots, budget, what, 1)
+        print(f"bench: cpu {key} {legs[key]['sample']}", file=sys.stderr, flush=True)
+    try:
+        from oracle.oracle import RefLib
+        refo = RefLib("libqcsref_omp.so")
+        for k, budget in ((1, 120), (nproc, 15)):
+            _omp_threads(k)
+            key = f"reference_openmp_{k}_threads"
+            legs[key] = _ref_leg(refo, n, ops, shots, budget,
+                                 "OpenMP build as written (racy), -O3", k)
+            print(f"bench: cpu {key} {legs[key]['sample']}", file=sys.stderr, flush=True)
+    except (FileNotFoundError, OSError):
+        pass
+    cl = qcs_amd.libc()
+    for compat in (False, True):
+        for k in (1, nproc):
+            st = OracleState(n, compat=compat)
+            st.L.oq_set_threads(k)
+            t0 = time.perf_counter()
+            st.run_omp(ops, racy=False)
+            dt = time.perf_counter() - t0
+            cl.srand(0)
+            u = np.array([cl.rand() / 2147483647.0 for _ in range(shots)])
+            t1 = time.perf_counter()
+            idx = st.sample(u)
+            ts = time.perf_counter() - t1
+            st.close()
+            sem = "as_written" if compat else "textbook"
+            key = f"oracle_openmp_race_fixed_{sem}_{k}_threads"
+            legs[key] = {"value": len(ops) * float(1 << n) / dt, "unit": "amplitude updates/s",
+                         "cores": k, "kind": "port", "gates_per_s": len(ops) / dt,
+                         "seconds": dt, "shots_s": ts, "shots_landed": int(np.sum(idx >= 0)),
+                         "sample": f"config 1 in full ({len(ops)} gates, {shots} shots), "
+                                   f"race-fixed OpenMP loops, {sem} controlled gate"}
+            print(f"bench: cpu {key} {dt:.2f} s", file=sys.stderr, flush=True)
+    _omp_threads(nproc)
+    head = legs.get("reference_O3") or next(iter(legs.values()))
+    print(json.dumps({
+        "metric": "amplitude updates/s, config 1 (20 qubits, depth 10, 810 gates) on host cores",
+        "value": head["value"], "unit": "amplitude updates/s",
+        "n_gpus": 0, "steps": 1, "warmup": 0,
+        "ms_per_step": head.get("seconds", 0.0) * 1e3, "higher_is_better": True,
+        "scaling": "strong", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "20-qubit layered Rz-Ry-Rz + CNOT-ladder circuit, depth 10, "
+                               "seed 0 (810 gates), then 1024 shots after srand(0)",
+                   "qubits": n, "gates": len(ops), "shots": shots},
+        "cpu": cpu_model(), "nproc": os.cpu_count(), "omp_threads": nproc,
+        "gcc": _tool_version(["gcc", "--version"]),
+        "glibc": platform.libc_ver()[1],
+        "legs": legs,
+    }), flush=True)
+
+
+def _tool_version(cmd):
+    try:
+        return subprocess.run(cmd, capture_output=True, text=True,
+                              timeout=10).stdout.splitlines()[0]
+    except (OSError, subprocess.SubprocessError, IndexError):
+        return None
 
 
 def _slice(ops, i):
@@ -644,6 +781,28 @@ def bench_batch(args):
           file=sys.stderr, flush=True)
     total_amp = amp_updates[False] + amp_updates[True]
     dev_ms = sum(totals) / len(totals)
+    # the complex128 part alone (it sets the launch time), for its roofline
+    c128 = [c for c, w in zip(circs, work) if not w[3]]
+    bp = qcs_amd.Batch()
+    bp.set_threads(args.batch_threads)
+    if not args.no_tune:
+        bp.autotune(c128)
+    cl.srand(3)
+    bp.sample(c128, shots)
+    c128_kernel_ms = bp.last_timing()[0]
+    bp.close()
+    # FP64 VALU floor of the complex128 circuits: every amplitude of a gate's
+    # sweep takes 6 FP64 operations for H, RY (real matrices: 4 multiplies, 2
+    # adds) and RZ (diagonal: one complex multiply), none for CNOT (an
+    # exchange); at one operation per lane per clock, 64 per CU per clock
+    # (an FMA would count 2 flops: 78.6 TFLOP/s), i.e. 39.3e12 per second
+    valu_peak = 64.0 * 256 * 2.4e9
+    ops64 = sum((n + 3 * n * d) * 6.0 * float(1 << n) for n, d, _, f in work if not f)
+    floor_ms = ops64 / valu_peak * 1e3
+    print(f"batch: complex128 part alone {c128_kernel_ms:.1f} ms (floor {floor_ms:.1f} ms)",
+          file=sys.stderr, flush=True)
+    parity = batch_parity(circs, work, shots, args)
+    print(f"batch: parity {parity}", file=sys.stderr, flush=True)
     result = {
         "metric": "amplitude updates/s, 4096 batched circuits (10-20 qubits), one persistent launch",
         "value": total_amp / T,
@@ -676,10 +835,115 @@ def bench_batch(args):
         "amp_updates_c64": amp_updates[True],
         "circuits_per_s": len(circs) / T,
         "record_s": t_record,
+        "roofline": {
+            "bound": "valu", "kernel": "k_batch, the complex128 circuits alone (FP64 VALU)",
+            "achieved": c128_kernel_ms, "peak": floor_ms, "unit": "ms",
+            "frac": floor_ms / c128_kernel_ms, "traffic": None,
+            "fp64_ops": ops64, "fp64_ops_peak_per_s": valu_peak,
+            "ops_per_amplitude": {"H": 6, "RY": 6, "RZ": 6, "CNOT": 0},
+        },
+        "parity": parity,
+        "cpu_baseline": None if args.no_cpu else ref_batch_baseline(work, shots, args.cpu_seconds),
     }
     for c in circs:
         c.close()
     print(json.dumps(result), flush=True)
+
+
+def batch_parity(circs, work, shots, args):
+    """One more run with every final state kept: each fused complex64
+    circuit against a complex128 twin batch of the same circuits and stream
+    (1 - F, and the samples that differ: an FP32 state moves a uniform
+    across a CDF boundary only when it lies within the two CDFs' difference
+    of it), and a spread sample of the complex128 circuits against circuits
+    of their own run gate by gate (amplitudes and samples bit for bit)."""
+    import numpy as np
+    import qcs_amd
+    from qcs_amd import circuits as C
+    cl = qcs_amd.libc()
+    b = qcs_amd.Batch(qcs_amd.BATCH_KEEP_STATES)
+    b.set_threads(args.batch_threads)
+    cl.srand(3)
+    idx = b.sample(circs, shots)
+    twins = []
+    for n, d, seed, fused in work:
+        c = qcs_amd.Circuit(n, qcs_amd.DEFERRED)
+        c.apply(_Arrays(C.layered_arrays(n, d, seed)))
+        twins.append(c)
+    t = qcs_amd.Batch(qcs_amd.BATCH_KEEP_STATES)
+    cl.srand(3)
+    tidx = t.sample(twins, shots)
+    worst, differ, total = 0.0, 0, 0
+    for i, (n, d, seed, fused) in enumerate(work):
+        if not fused:
+            continue
+        a, r = b.amplitudes(i, n), t.amplitudes(i, n)
+        f = 1.0 - abs(np.vdot(a, r)) ** 2 / (np.vdot(a, a).real * np.vdot(r, r).real)
+        worst = max(worst, f)
+        differ += int(np.sum(idx[i] != tidx[i]))
+        total += shots
+    # every 64th complex128 circuit (and the largest), run on a state of its own
+    plain = [i for i, w in enumerate(work) if not w[3]]
+    pick = sorted(set(plain[::64]) | {max(plain, key=lambda i: (work[i][0], work[i][1]))})
+    checked, equal = 0, 0
+    cl.srand(3)
+    for i, (n, d, seed, fused) in enumerate(work):
+        if i not in pick:
+            for _ in range(shots):
+                cl.rand()
+            continue
+        with qcs_amd.Circuit(n) as c:
+            c.apply(_Arrays(C.layered_arrays(n, d, seed)))
+            own = c.sample_indices(shots)
+            ok = np.array_equal(c.amplitudes(), b.amplitudes(i, n)) and \
+                np.array_equal(own, idx[i])
+        checked += 1
+        equal += int(ok)
+        if i == pick[-1]:
+            break
+    for c in twins:
+        c.close()
+    b.close()
+    t.close()
+    return {"c64_worst_one_minus_f": worst, "c64_samples_differing_from_c128": differ,
+            "c64_samples": total,
+            "c128_circuits_checked_bit_exact": checked, "c128_circuits_equal": equal}
+
+
+def ref_batch_baseline(work, shots, seconds):
+    """The reference's own path per circuit (qc_create, the gates one call
+    each, qc_run_shots), sequential build, 1 core, on the workload's circuits
+    in order, skipping those that would not finish in the time left."""
+    import ctypes
+    ref = _reflib()
+    if ref is None:
+        return None
+    from qcs_amd import circuits as C
+    import qcs_amd
+    qcs_amd.libc().srand(3)
+    done, amp, skipped, t0 = 0, 0.0, 0, time.perf_counter()
+    ns_per_amp = 30e-9  # the reference's measured rate on these hosts, for the skip test
+    for n, d, seed, fused in work:
+        left = seconds - (time.perf_counter() - t0)
+        g = C.gate_count_layered(n, d)
+        if left <= 0:
+            break
+        if g * float(1 << n) * ns_per_amp > left:
+            skipped += 1
+            continue
+        c = ref.create(n)
+        ref.apply(c, _Arrays(C.layered_arrays(n, d, seed)))
+        hist = (ctypes.c_int * (1 << n))()
+        ref.L.qc_run_shots(c, shots, hist)
+        ref.L.qc_destroy(c)
+        done += 1
+        amp += g * float(1 << n)
+    dt = time.perf_counter() - t0
+    return {"value": amp / dt, "unit": "amplitude updates/s", "cores": 1, "kind": "reference",
+            "circuits_per_s": done / dt,
+            "sample": f"{done} circuits of the workload in order ({skipped} skipped as too "
+                      f"long for the {seconds:.0f} s budget), gates + qc_run_shots({shots}) "
+                      "each, reference sequential build"}
 
 
 def run_mode(hip, ops, n, mode, args, steps, warmup, tune):
@@ -695,15 +959,33 @@ def run_mode(hip, ops, n, mode, args, steps, warmup, tune):
     if mode == "split":
         opts |= qcs_amd.FUSE_SPLIT_BF16
     tiled = mode in ("tiles", "split")
+    import ctypes
     fused = qcs_amd.Circuit(n, opts)
     hip.qcs_hip_set_tuning(fused.device_state, qcs_amd.TUNE_TILE_THREADS, args.tile_threads)
+    if args.k4_form:
+        hip.qcs_hip_set_tuning(fused.device_state, qcs_amd.TUNE_TILE_K4_FORM,
+                               qcs_amd.TILE_K4_HALVES if args.k4_form == "halves"
+                               else qcs_amd.TILE_K4_CHAIN)
     fuse_k = args.fuse_k or (4 if tiled else 5)
-    autotuned = tune and not args.fuse_k and not tiled
+    autotuned = tune and not args.fuse_k and not args.k4_form
     if autotuned:
-        # kernel variant per (n, k) and fusion width for this circuit
+        # one pass per block: kernel variant per (n, k) and the fusion width;
+        # tile programs: fusion width, work-group size and grid timed on this
+        # circuit's own pass plan (cached per device and gate structure)
         fused.apply(ops)
+        t0 = time.perf_counter()
         fuse_k = fused.autotune()
+        tune_s = time.perf_counter() - t0
+        if fuse_k < 1:
+            raise RuntimeError(f"qc_autotune failed ({fuse_k})")
         fused.flush()
+    knob = ctypes.c_int()
+    hip.qcs_hip_get_tuning(fused.device_state, qcs_amd.TUNE_TILE_THREADS, ctypes.byref(knob))
+    tile_threads = knob.value
+    hip.qcs_hip_get_tuning(fused.device_state, qcs_amd.TUNE_TILE_GRID, ctypes.byref(knob))
+    tile_grid = knob.value
+    hip.qcs_hip_get_tuning(fused.device_state, qcs_amd.TUNE_TILE_K4_FORM, ctypes.byref(knob))
+    k4_form = "halves" if knob.value == qcs_amd.TILE_K4_HALVES else "chain"
 
     def step():
         fused.set_basis_state(0)
@@ -734,7 +1016,10 @@ def run_mode(hip, ops, n, mode, args, steps, warmup, tune):
         "gates_per_s": G / T,
         "fusion_k": fuse_k,
         "autotuned": autotuned,
-        "tile_threads": args.tile_threads,
+        "autotune_s": tune_s if autotuned else None,
+        "tile_threads": tile_threads,
+        "tile_grid": tile_grid,
+        "k4_form": k4_form,
         "fused_blocks": (fused.get_num_fused_blocks() - b0) // steps,
         "hbm_passes": (fused.get_num_fused_passes() - p0) // steps,
         "fused_kernel_ms_per_step": f_ms / steps,
@@ -765,6 +1050,9 @@ def main():
         sys.exit(2)
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.workload == "cpu":
+        bench_cpu(args)
         return
     if args.workload == "batch":
         bench_batch(args)
@@ -818,7 +1106,10 @@ def main():
             "fusion_k": m["fusion_k"],
             "autotuned": m["autotuned"],
             "fusion_mode": args.mode,
-            "tile_threads": args.tile_threads,
+            "tile_threads": m["tile_threads"],
+            "tile_grid": m["tile_grid"],
+            "k4_form": m["k4_form"],
+            "autotune_s": m["autotune_s"],
             "fused_blocks": m["fused_blocks"],
             "hbm_passes": m["hbm_passes"],
         },

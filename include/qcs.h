@@ -206,11 +206,16 @@ int qc_batch_set_threads(t_q_batch *batch, int threads);
 /* Autotuning, cached per device in a text file ($QCS_TUNE_CACHE, a path or
  * "off"; else $XDG_CACHE_HOME/qcs_amd/tune.txt or
  * ~/.cache/qcs_amd/tune.txt).
- * qc_autotune: fused circuit -- times the fused-pass kernel variants of
- * every width k on the circuit's state (identity blocks, the state is left
- * as it is), keeps the fastest per (n, k) and picks the fusion width for
- * the recorded gates (blocks x time per pass).  Returns that width or a
- * negative QCS_HIP_E* code. */
+ * qc_autotune: fused circuit, one pass per block -- times the fused-pass
+ * kernel variants of every width k on the circuit's state (identity blocks,
+ * the state is left as it is), keeps the fastest per (n, k) and picks the
+ * fusion width for the recorded gates (blocks x time per pass).  Fused
+ * circuit as tile programs (QCS_FUSE_TILES / QCS_FUSE_SPLIT_BF16, n >= 16)
+ * -- times the recorded gates' own pass plan with identity blocks at fusion
+ * widths 3 and 4, 512 or 1024 threads per work-group, one or two
+ * work-groups per CU and both forms of k = 4 blocks (QCS_HIP_TUNE_TILE_K4_FORM),
+ * and keeps the one with the least total pass time.
+ * Returns the fusion width or a negative QCS_HIP_E* code. */
 int qc_autotune(t_q_circuit *circuit);
 /* qc_batch_autotune: for every (precision, n) of the given deferred
  * circuits whose state is streamed through on-chip tiles, times candidate

@@ -124,6 +124,13 @@ int qcs_hip_qubit_probs(qcs_hip_state *s, int qubit, double *p0, double *p1);
  * supplied by the host, outcome = (u <= p0) ? 0 : 1; zero the other half and
  * normalize.  Returns the outcome in *outcome. */
 int qcs_hip_measure(qcs_hip_state *s, int qubit, double u, int *outcome);
+/* qc_measure_all's loop (src/qcs.c:296-302): qcs_hip_measure of qubits[0],
+ * qubits[1], ... in order with the uniforms u[0], u[1], ..., the outcome of
+ * each decided on the device, so the whole sequence needs one host
+ * synchronisation instead of one per qubit.  The same reductions give the
+ * same p0 and p1 as qcs_hip_measure. */
+int qcs_hip_measure_seq(qcs_hip_state *s, int count, const int *qubits,
+                        const double *u, int *outcomes);
 /* |a_index|^2 computed as re*re + im*im (qc_get_probability,
  * src/qcs.c:391-395) */
 int qcs_hip_probability(qcs_hip_state *s, long index, double *p);
@@ -236,6 +243,15 @@ int qcs_hip_apply_passes(qcs_hip_state *s, const qcs_hip_pass *passes,
                                           accumulation: the state is rounded
                                           to 16 significant bits per block
                                           (1 - F per block ~1e-11) */
+/* QCS_HIP_TUNE_TILE_K4_FORM: how FP32 tile programs on 1024 threads run
+ * blocks of 4 qubits, one of: */
+#define QCS_HIP_TUNE_TILE_K4_FORM 3
+#define QCS_HIP_TILE_K4_CHAIN 0  /* one 32x32 matrix-core chain per wave
+                                    (v_mfma_f32_32x32x2f32) */
+#define QCS_HIP_TILE_K4_HALVES 1 /* two 16-column halves on
+                                    v_mfma_f32_16x16x4f32, software pipelined:
+                                    a half's LDS reads and stores, and the next
+                                    block's, overlap the other half's chain */
 /* QCS_HIP_TUNE_BLOCK_VARIANT(k): launch variant of the one-block-per-pass
  * matrix-core kernel for blocks of k qubits (1..5), one of: */
 #define QCS_HIP_TUNE_BLOCK_VARIANT(k) (8 + (k))
@@ -245,6 +261,8 @@ int qcs_hip_apply_passes(qcs_hip_state *s, const qcs_hip_pass *passes,
 #define QCS_HIP_BLOCK_TB13_512_PIPE 3 /* 8192-amplitude tiles, 512 threads
                                          (default) */
 int qcs_hip_set_tuning(qcs_hip_state *s, int knob, int value);
+/* the current value of a knob (what the autotuner chose, or the default) */
+int qcs_hip_get_tuning(qcs_hip_state *s, int knob, int *value);
 
 /* ---- comparison ----------------------------------------------------------
  * FP64 on the device: one_minus_f = 1 - |<a|b>|^2 / (|a|^2 |b|^2) and the

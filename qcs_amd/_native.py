@@ -51,7 +51,8 @@ class HipPass(ctypes.Structure):
 
 
 TILE_BITS, TILE_LOW_BITS = 13, 4
-TUNE_TILE_THREADS, TUNE_TILE_GRID, TUNE_TILE_MATH = 0, 1, 2
+TUNE_TILE_THREADS, TUNE_TILE_GRID, TUNE_TILE_MATH, TUNE_TILE_K4_FORM = 0, 1, 2, 3
+TILE_K4_CHAIN, TILE_K4_HALVES = 0, 1
 TILE_MATH_FP32, TILE_MATH_SPLIT_BF16 = 0, 1
 FUSE_TILES = 0x10
 FUSE_SPLIT_BF16 = 0x40
@@ -95,6 +96,7 @@ _HIP_PROTOS = {
     "qcs_hip_normalize": (c_int, [c_void_p, P(c_double)]),
     "qcs_hip_qubit_probs": (c_int, [c_void_p, c_int, P(c_double), P(c_double)]),
     "qcs_hip_measure": (c_int, [c_void_p, c_int, c_double, P(c_int)]),
+    "qcs_hip_measure_seq": (c_int, [c_void_p, c_int, P(c_int), P(c_double), P(c_int)]),
     "qcs_hip_probability": (c_int, [c_void_p, c_long, P(c_double)]),
     "qcs_hip_argmax": (c_int, [c_void_p, P(c_long)]),
     "qcs_hip_sample": (c_int, [c_void_p, c_long, P(c_double), P(c_long)]),
@@ -105,6 +107,7 @@ _HIP_PROTOS = {
     "qcs_hip_apply_passes": (c_int, [c_void_p, P(HipPass), c_int, P(HipBlock), c_int,
                                      P(HipGate), c_int]),
     "qcs_hip_set_tuning": (c_int, [c_void_p, c_int, c_int]),
+    "qcs_hip_get_tuning": (c_int, [c_void_p, c_int, P(c_int)]),
     "qcs_hip_timing_enable": (c_int, [c_void_p, c_int]),
     "qcs_hip_timing_read": (c_int, [c_void_p, c_int, P(c_double), P(c_long),
                                     P(c_double), P(c_double)]),

@@ -1,7 +1,10 @@
 // extern "C" entry points of the backend (declared in include/qcs_hip.h):
 // argument validation, state lifetime, copies, kernel timers, dispatch.
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
+#include <cstdlib>
+#include <mutex>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -30,8 +33,34 @@ int check_hip(hipError_t e, const char *what) {
   return fail(code, "%s: %s", what, hipGetErrorString(e));
 }
 
+// Fault injection for tests of the error path: QCS_HIP_INJECT_FAULT=launch:N
+// makes the N-th launch check of the process (1-based) report a failure, and
+// alloc:N the N-th state allocation.  It is decided on the host: the kernel
+// still runs (or the allocation is never made), so the device never faults.
+enum { kInjectNone = 0, kInjectLaunch = 1, kInjectAlloc = 2 };
+static int g_inject_kind = kInjectNone;
+static long g_inject_at = 0;
+static std::atomic<long> g_inject_count[3];
+static std::once_flag g_inject_once;
+
+static bool inject_fault(int kind) {
+  std::call_once(g_inject_once, [] {
+    const char *e = std::getenv("QCS_HIP_INJECT_FAULT");
+    if (!e) return;
+    if (std::strncmp(e, "launch:", 7) == 0) g_inject_kind = kInjectLaunch;
+    else if (std::strncmp(e, "alloc:", 6) == 0) g_inject_kind = kInjectAlloc;
+    else return;
+    g_inject_at = std::atol(std::strchr(e, ':') + 1);
+  });
+  if (kind != g_inject_kind || g_inject_at <= 0) return false;
+  return g_inject_count[kind].fetch_add(1) + 1 == g_inject_at;
+}
+
 int launch_check(const char *what) {
-  return check_hip(hipGetLastError(), what);
+  const hipError_t e = hipGetLastError();
+  if (e == hipSuccess && inject_fault(kInjectLaunch))
+    return fail(QCS_HIP_EDEVICE, "%s: injected failure (QCS_HIP_INJECT_FAULT)", what);
+  return check_hip(e, what);
 }
 
 static hipEvent_t take_event(qcs_hip_state *s) {
@@ -259,6 +288,8 @@ int qcs_hip_state_create(qcs_hip_state **out, int num_qubits, int precision,
   if ((e = hipGetDevice(&s->device)) != hipSuccess) rc = check_hip(e, "hipGetDevice");
   if (!rc && (e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking)) != hipSuccess)
     rc = check_hip(e, "hipStreamCreate");
+  if (!rc && inject_fault(kInjectAlloc))
+    rc = fail(QCS_HIP_ENOMEM, "hipMalloc(state): injected failure (QCS_HIP_INJECT_FAULT)");
   if (!rc && (e = hipMalloc(&s->vec, s->size * s->amp_bytes)) != hipSuccess)
     rc = check_hip(e, "hipMalloc(state)");
   if (!rc && (flags & QCS_HIP_COMPAT) &&
@@ -447,6 +478,17 @@ int qcs_hip_measure(qcs_hip_state *s, int qubit, double u, int *outcome) {
   return 0;
 }
 
+int qcs_hip_measure_seq(qcs_hip_state *s, int count, const int *qubits,
+                        const double *u, int *outcomes) {
+  int rc;
+  if ((rc = check_state(s))) return rc;
+  if (count <= 0) return 0;
+  if (!qubits || !u || !outcomes) return fail(QCS_HIP_EINVAL, "null buffer");
+  for (int i = 0; i < count; ++i)
+    if ((rc = check_qubit(s, qubits[i], "measure_seq"))) return rc;
+  return measure_seq(s, count, qubits, u, outcomes);
+}
+
 int qcs_hip_probability(qcs_hip_state *s, long index, double *p) {
   int rc;
   if ((rc = check_state(s))) return rc;
@@ -600,6 +642,12 @@ int qcs_hip_set_tuning(qcs_hip_state *s, int knob, int value) {
     s->tile_math = value;
     return 0;
   }
+  if (knob == QCS_HIP_TUNE_TILE_K4_FORM) {
+    if (value != QCS_HIP_TILE_K4_CHAIN && value != QCS_HIP_TILE_K4_HALVES)
+      return fail(QCS_HIP_EINVAL, "unknown k = 4 tile form %d", value);
+    s->tile_k4 = value;
+    return 0;
+  }
   if (knob == QCS_HIP_TUNE_TILE_GRID) {
     if (value < 8 || value > 65536 || (value & 7))
       return fail(QCS_HIP_EINVAL, "tile grid must be a multiple of 8 in [8, 65536]");
@@ -607,6 +655,22 @@ int qcs_hip_set_tuning(qcs_hip_state *s, int knob, int value) {
     return 0;
   }
   return fail(QCS_HIP_EINVAL, "unknown tuning knob %d", knob);
+}
+
+int qcs_hip_get_tuning(qcs_hip_state *s, int knob, int *value) {
+  int rc;
+  if ((rc = check_state(s))) return rc;
+  if (!value) return fail(QCS_HIP_EINVAL, "null output");
+  if (knob == QCS_HIP_TUNE_TILE_THREADS) *value = s->tile_threads;
+  else if (knob == QCS_HIP_TUNE_TILE_GRID) *value = s->tile_grid;
+  else if (knob == QCS_HIP_TUNE_TILE_MATH) *value = s->tile_math;
+  else if (knob == QCS_HIP_TUNE_TILE_K4_FORM) *value = s->tile_k4;
+  else if (knob >= QCS_HIP_TUNE_BLOCK_VARIANT(1) &&
+           knob <= QCS_HIP_TUNE_BLOCK_VARIANT(QCS_HIP_MAX_FUSE))
+    *value = s->block_variant[knob - QCS_HIP_TUNE_BLOCK_VARIANT(0)];
+  else
+    return fail(QCS_HIP_EINVAL, "unknown tuning knob %d", knob);
+  return 0;
 }
 
 int qcs_hip_compare(qcs_hip_state *a, qcs_hip_state *b, double *one_minus_f,

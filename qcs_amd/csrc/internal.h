@@ -73,6 +73,7 @@ struct qcs_hip_state {
   int tile_threads = 1024;  // work-group size of the tile-program kernel
   int tile_grid = 256;      // persistent grid of the tile-program kernel
   int tile_math = QCS_HIP_TILE_MATH_FP32;  // block arithmetic of tile programs
+  int tile_k4 = QCS_HIP_TILE_K4_CHAIN;     // form of k = 4 blocks in tile programs
   // kernel timing
   int timing = 0;
   std::vector<qcs::TimedLaunch> pending_events;
@@ -129,6 +130,10 @@ int qubit_probs(qcs_hip_state *s, int q, double *p0, double *p1);
 bool half_prob_pays(const qcs_hip_state *s, int q);
 int half_prob(qcs_hip_state *s, int q, int bit, double *p);
 int collapse(qcs_hip_state *s, int q, int outcome, double factor, int do_scale);
+// measurements of qubits[0..count) in order with the uniforms u, decided on
+// the device, one host synchronisation
+int measure_seq(qcs_hip_state *s, int count, const int *qubits, const double *u,
+                int *outcomes);
 int argmax(qcs_hip_state *s, long *idx);
 int sample(qcs_hip_state *s, long shots, const double *u, long *out);
 // qcs_hip_sample's kernels on caller-supplied device buffers, with the chunk

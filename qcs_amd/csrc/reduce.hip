@@ -160,9 +160,44 @@ __global__ __launch_bounds__(kThreads) void k_scale_dev(
 // qc_measure's collapse: zero the amplitudes whose bit q differs from
 // `outcome`, scale the others by f when do_scale.
 template <typename T>
+__device__ __forceinline__ void collapse_body(typename Amp<T>::type *a, uint64_t n, int q,
+                                              int outcome, double f, int do_scale);
+
+template <typename T>
 __global__ __launch_bounds__(kThreads) void k_collapse(
     typename Amp<T>::type *a, uint64_t n, int q, int outcome, double f,
     int do_scale) {
+  collapse_body<T>(a, n, q, outcome, f, do_scale);
+}
+
+// qc_measure's decision on the device (src/qcs.c:261-283): m[0], m[1] hold
+// p0 and p1; outcome = (u <= p0) ? 0 : 1, and the kept half, whose norm is
+// p0 or p1, is rescaled by 1/sqrt of it when that is > 1e-12 and != 1 (the
+// rule of q_state_normalize).  m[2..4] = outcome, factor, rescale flag; the
+// outcome also goes to *record.
+__global__ void k_measure_decide(double *m, double u, double *record) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double p0 = m[0], p1 = m[1];
+  const int o = (u <= p0) ? 0 : 1;
+  const double ns = o == 0 ? p0 : p1;
+  const int sc = (ns > 1e-12 && ns != 1.0) ? 1 : 0;
+  m[2] = double(o);
+  m[3] = sc ? 1.0 / sqrt(ns) : 1.0;
+  m[4] = double(sc);
+  *record = double(o);
+}
+
+// k_collapse with the outcome, factor and flag k_measure_decide wrote
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_collapse_dev(typename Amp<T>::type *a,
+                                                           uint64_t n, int q,
+                                                           const double *m) {
+  collapse_body<T>(a, n, q, int(m[2]), m[3], int(m[4]));
+}
+
+template <typename T>
+__device__ __forceinline__ void collapse_body(typename Amp<T>::type *a, uint64_t n, int q,
+                                              int outcome, double f, int do_scale) {
   const uint64_t nvb = ceil_div(n, uint64_t(kThreads));
   for (uint64_t vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
     const uint64_t i = xcd_order(vb, nvb) * kThreads + threadIdx.x;
@@ -741,6 +776,63 @@ int collapse(qcs_hip_state *s, int q, int outcome, double f, int do_scale) {
   return s->precision == QCS_HIP_C128
              ? collapse_impl<double>(s, q, outcome, f, do_scale)
              : collapse_impl<float>(s, q, outcome, f, do_scale);
+}
+
+// Measurements of `count` qubits in order, with no host round trip between
+// them: per qubit the p0 / p1 reductions qcs_hip_measure uses (the half
+// passes, or one pass for both where the halves' runs are shorter than a
+// line; here p1 is always formed, as the device decides the outcome),
+// k_measure_decide, and the collapse reading its decision.  Result slots:
+// 8..12 the current measurement, 16.. the outcomes.
+template <typename T>
+static int measure_seq_impl(qcs_hip_state *s, int count, const int *qubits,
+                            const double *u, int *outcomes) {
+  typedef typename Amp<T>::type A;
+  double *m = result_slot(s) + 8, *rec = result_slot(s) + 16;
+  constexpr int kMaxRec = 48;
+  int rc;
+  for (int i0 = 0; i0 < count; i0 += kMaxRec) {
+    const int c = std::min(kMaxRec, count - i0);
+    for (int i = 0; i < c; ++i) {
+      const int q = qubits[i0 + i];
+      if (half_prob_pays(s, q)) {
+        const unsigned nb = red_blocks(s->size / 2);
+        for (int bit = 0; bit < 2; ++bit) {
+          if ((rc = half_partials<T>(s, q, bit, nb))) return rc;
+          hipLaunchKernelGGL(k_final, dim3(1), dim3(kThreads), 0, s->stream, s->red,
+                             int(nb), 1, 0u, m + bit);
+          if ((rc = launch_check("k_final"))) return rc;
+        }
+      } else {
+        const unsigned nb = red_blocks(s->size);
+        if ((rc = norm_partials<T>(s, q, nb))) return rc;
+        hipLaunchKernelGGL(k_final, dim3(1), dim3(kThreads), 0, s->stream, s->red,
+                           int(nb), 2, 0u, m);
+        if ((rc = launch_check("k_final"))) return rc;
+      }
+      hipLaunchKernelGGL(k_measure_decide, dim3(1), dim3(64), 0, s->stream, m,
+                         u[i0 + i], rec + i);
+      if ((rc = launch_check("k_measure_decide"))) return rc;
+      TimedLaunch tl;
+      timing_begin(s, QCS_HIP_K_OTHER, 1.5 * double(s->size) * sizeof(A), 0.0, &tl);
+      hipLaunchKernelGGL(k_collapse_dev<T>, dim3(elem_grid(s->size)), dim3(kThreads), 0,
+                         s->stream, static_cast<A *>(s->vec), s->size, q, m);
+      if ((rc = launch_check("k_collapse_dev"))) return rc;
+      if ((rc = timing_end(s, &tl))) return rc;
+    }
+    QCS_HIP_TRY(hipMemcpyAsync(s->host, rec, sizeof(double) * size_t(c),
+                               hipMemcpyDeviceToHost, s->stream));
+    QCS_HIP_TRY(hipStreamSynchronize(s->stream));
+    for (int i = 0; i < c; ++i) outcomes[i0 + i] = int(s->host[i]);
+  }
+  return 0;
+}
+
+int measure_seq(qcs_hip_state *s, int count, const int *qubits, const double *u,
+                int *outcomes) {
+  return s->precision == QCS_HIP_C128
+             ? measure_seq_impl<double>(s, count, qubits, u, outcomes)
+             : measure_seq_impl<float>(s, count, qubits, u, outcomes);
 }
 
 template <typename T>

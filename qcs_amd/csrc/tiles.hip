@@ -62,6 +62,7 @@
 //
 // Work-group tile order is XCD aware: the hardware places work-group i on
 // XCD i % 8, so XCD x is given the contiguous x-th eighth of the tiles.
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -118,7 +119,10 @@ struct TileBlock {
   int wslot;
   int sync;                  // 1: barrier after this block (the next block's
                              //    waves own other amplitudes)
-  int reserved;
+  int early;                 // 1: k = 4 in two pipelined column halves, and
+                             //    the next block is too, in the same run and
+                             //    with the same half bit: its first half's
+                             //    B operands are read during this block
   int rowoff[kTileRMax];     // 4 swz(row bits of s): byte offset of row s
   int wpart[16];             // 4 swz(wave index on the wave bits)
   int jpart[16];             // 4 swz(j on the group bits)
@@ -174,6 +178,7 @@ __device__ __forceinline__ void split_pair(float x0, float x1, unsigned &hi,
 // 4 and 8 give all of them.
 struct BlockScalars {
   int k, sync, up;  // up: the wave's part of the column address
+  int early;
   int rb[kTileKMax];
   __device__ __forceinline__ int ro(int s) const {
     int o = 0;
@@ -191,6 +196,7 @@ __device__ __forceinline__ BlockScalars load_scalars(const TileBlock *__restrict
   BlockScalars d;
   d.k = __builtin_amdgcn_readfirstlane(bl->k);
   d.sync = __builtin_amdgcn_readfirstlane(bl->sync);
+  d.early = __builtin_amdgcn_readfirstlane(bl->early);
   d.up = __builtin_amdgcn_readfirstlane(bl->wpart[wave]);
 #pragma unroll
   for (int q = 0; q < kTileKMax; ++q)
@@ -381,6 +387,111 @@ __device__ __forceinline__ void tp_block16(const TileBlock *__restrict__ bl,
   }
 }
 
+// A k = 4 block on v_mfma_f32_16x16x4f32 as two 16-column halves, software
+// pipelined (1024 threads: one 32-column group per wave).  The 32x32 chain
+// of tp_block leaves the matrix cores idle while a wave stores its results
+// and reads the next block's operands, and the waves of a SIMD do that at
+// the same time.  Here a wave keeps its own matrix-core queue filled:
+//   half-0 chain | previous block's half-1 stores, half-1 B reads, next
+//                |   block's scalars
+//   half-1 chain | next block's A operands and lane table, half-0 stores,
+//                |   and (early) the next block's half-0 B reads
+// The half-1 results are stored at the start of the next block, behind its
+// half-0 chain, when that block is a k = 4 block of the same run with the
+// same half bit (lane bit 4): its first half then reads only amplitudes that
+// this block's first half wrote, or that no block of the run wrote, so its
+// B reads go out before this block's second half is stored.  Output rows of
+// row block rb are those of tp_block16 with target bit 3 = rb; the A operands
+// of instruction sp of row block rb sit at index 8 rb + sp.
+struct K4State {
+  float bv0[8];  // first-half B operands of the current block, read ahead
+  f4v acc[2];    // second-half results of the previous block, not yet stored
+  int po, pr1, pr3;  // their base address, row offsets of target bits 1, 3
+  int have_b0, pend;
+};
+
+// One column half's two 16x16 accumulator tiles (row blocks 0 and 1) to LDS.
+// Lane (q, c) holds rows (q & 1) | (v >> 1) << 1 | (q >> 1) << 2 | rb << 3,
+// part v & 1; `o` is the lane's column ^ the wave part ^ the row offsets q
+// selects.  Every term is below the plane offset, so re and im pairs are one
+// two-address store.
+__device__ __forceinline__ void store_k4_half(int o, int r1, int r3, const f4v &a0,
+                                              const f4v &a1) {
+#pragma unroll
+  for (int rbk = 0; rbk < 2; ++rbk) {
+#pragma unroll
+    for (int vh = 0; vh < 2; ++vh) {
+      const int addr = o ^ (vh ? r1 : 0) ^ (rbk ? r3 : 0);
+      lds_store<float>(addr, (rbk ? a1 : a0)[2 * vh]);
+      lds_store<float>(addr + kTilePlane, (rbk ? a1 : a0)[2 * vh + 1]);
+    }
+  }
+}
+
+template <int NTH, int TILE>
+__device__ __forceinline__ void tp_block4p(const BlockScalars &d, const BlockRegs &r,
+                                           const TileBlock *__restrict__ nbl, int nblk,
+                                           int wave, BlockScalars &nd, BlockRegs &nr,
+                                           K4State &st) {
+  static_assert((TILE >> 4) / 32 / (NTH / 64) == 1, "one 32-column group per wave");
+  const int lane = threadIdx.x & 63, q = lane >> 4;
+  float wa[16];
+#pragma unroll
+  for (int s = 0; s < 16; ++s) wa[s] = r.a[s >> 2][s & 3];
+  int ro2[8];
+#pragma unroll
+  for (int sp = 0; sp < 8; ++sp) ro2[sp] = d.ro(2 * sp);
+  const int lc0 = (lane & 16) ? r.lp2 : r.lp, lc1 = (lane & 16) ? r.lp : r.lp2;
+  const int rq = (q & 1) ? d.rb[0] : 0;
+  const int bq = rq ^ ((q >> 1) ? kTilePlane : 0) ^ d.up;  // B: row 2sp + (q & 1), part q >> 1
+  const int oq = rq ^ ((q >> 1) ? d.rb[2] : 0) ^ d.up;     // results: rows (q & 1) | (q >> 1) << 2
+  if (!st.have_b0) {
+#pragma unroll
+    for (int sp = 0; sp < 8; ++sp) st.bv0[sp] = lds_load<float>(lc0 ^ bq ^ ro2[sp]);
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  const f4v z = {0.f, 0.f, 0.f, 0.f};
+  f4v a00 = z, a01 = z;
+#pragma unroll
+  for (int sp = 0; sp < 8; ++sp) {
+    a00 = __builtin_amdgcn_mfma_f32_16x16x4f32(wa[sp], st.bv0[sp], a00, 0, 0, 0);
+    a01 = __builtin_amdgcn_mfma_f32_16x16x4f32(wa[8 + sp], st.bv0[sp], a01, 0, 0, 0);
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  if (st.pend) store_k4_half(st.po, st.pr1, st.pr3, st.acc[0], st.acc[1]);
+  float bv1[8];
+#pragma unroll
+  for (int sp = 0; sp < 8; ++sp) bv1[sp] = lds_load<float>(lc1 ^ bq ^ ro2[sp]);
+  nd = load_scalars(nbl, wave);
+  __builtin_amdgcn_sched_barrier(0);
+  f4v a10 = z, a11 = z;
+#pragma unroll
+  for (int sp = 0; sp < 8; ++sp) {
+    a10 = __builtin_amdgcn_mfma_f32_16x16x4f32(wa[sp], bv1[sp], a10, 0, 0, 0);
+    a11 = __builtin_amdgcn_mfma_f32_16x16x4f32(wa[8 + sp], bv1[sp], a11, 0, 0, 0);
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  load_regs(nr, nblk);
+  store_k4_half(lc0 ^ oq, d.rb[1], d.rb[3], a00, a01);
+  if (d.early) {
+    const int nlc0 = (lane & 16) ? nr.lp2 : nr.lp;
+    const int nbq = ((q & 1) ? nd.rb[0] : 0) ^ ((q >> 1) ? kTilePlane : 0) ^ nd.up;
+#pragma unroll
+    for (int sp = 0; sp < 8; ++sp) st.bv0[sp] = lds_load<float>(nlc0 ^ nbq ^ nd.ro(2 * sp));
+    st.have_b0 = 1;
+    st.acc[0] = a10;
+    st.acc[1] = a11;
+    st.po = lc1 ^ oq;
+    st.pr1 = d.rb[1];
+    st.pr3 = d.rb[3];
+    st.pend = 1;
+  } else {
+    store_k4_half(lc1 ^ oq, d.rb[1], d.rb[3], a10, a11);
+    st.have_b0 = 0;
+    st.pend = 0;
+  }
+}
+
 // One FP32 block of the pass (index b), with the next block's values read
 // into (nd, nr); the block after the last is the pass's first (next tile).
 template <int NTH, int TILE>
@@ -474,7 +585,7 @@ __device__ __forceinline__ void tp_block_split(const TileBlock *__restrict__ bl,
 // The descriptor pointers are __restrict__ so the compiler may read them
 // through the scalar cache (they are uniform and never written by the
 // kernel); vector loads of them would wait behind the tile prefetch.
-template <int TB, int NTH, bool SPLIT>
+template <int TB, int NTH, bool SPLIT, bool K4P>
 __global__ __launch_bounds__(NTH) void k_tile_program(
     float2 *__restrict__ a, const TilePass *__restrict__ passes, int pass_index,
     const TileBlock *__restrict__ blocks, const float *__restrict__ wbuf,
@@ -503,12 +614,14 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
     const int R = 1 << bl.k, RR = 2 * R, row = ln & 31, hh = ln >> 5;
     const float *W = wbuf + size_t(bl.wslot) * kTileWFloats;
     float w = (row < RR && s < R) ? W[row * RR + 2 * s + hh] : 0.0f;
-    if (!SPLIT && bl.k <= 3) {
-      // the 16-column form (tp_block16): instruction s, A[i][qq] with i =
-      // 4 qi + vi holding output row (qi & 1) | (vi >> 1) << 1 | (qi >> 1) << 2,
-      // part vi & 1, and qq meaning input row 2s + (qq & 1), part qq >> 1
-      const int i = ln & 15, qq = ln >> 4, qi = i >> 2, vi = i & 3;
-      const int ro = (qi & 1) | ((vi >> 1) << 1) | ((qi >> 1) << 2), si = 2 * s + (qq & 1);
+    if (!SPLIT && (bl.k <= 3 || K4P)) {
+      // the 16-column form (tp_block16, tp_block4p): instruction sp = s & 7
+      // of row block rb = s >> 3, A[i][qq] with i = 4 qi + vi holding output
+      // row (qi & 1) | (vi >> 1) << 1 | (qi >> 1) << 2 | rb << 3, part vi & 1,
+      // and qq meaning input row 2 sp + (qq & 1), part qq >> 1
+      const int i = ln & 15, qq = ln >> 4, qi = i >> 2, vi = i & 3, rb = s >> 3, sp = s & 7;
+      const int ro = (qi & 1) | ((vi >> 1) << 1) | ((qi >> 1) << 2) | (rb << 3);
+      const int si = 2 * sp + (qq & 1);
       w = (ro < R && si < R) ? W[(2 * ro + (vi & 1)) * RR + 2 * si + (qq >> 1)] : 0.0f;
     }
     const int blk = OPS + b * STRIDE;
@@ -635,15 +748,37 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
       // two register sets taken in turn, so the values read one block ahead
       // need no copies; (sa, ra) hold the pass's first block on entry
       const TileBlock *fb = &blocks[ps.first_block];
-      for (int b = 0;;) {
-        tp_step<NTH, TILE>(fb, b, nb, OPS, wave, sa, ra, sb, rb);
-        if (++b == nb) {
-          sa = sb;
-          ra = rb;
-          break;
+      if constexpr (K4P) {
+        // every block is a k = 4 block (the host pads smaller ones); one set
+        // of per-lane values, read in place once a block's last chain is
+        // issued; the scalars alternate between two sets
+        K4State st;
+        st.have_b0 = 0;
+        st.pend = 0;
+        for (int b = 0;;) {
+          int bn = b + 1 < nb ? b + 1 : 0;
+          tp_block4p<NTH, TILE>(sa, ra, fb + bn, OPS + bn * kTileStrideFp32, wave, sb, ra, st);
+          if (sa.sync) __syncthreads();
+          if (++b == nb) {
+            sa = sb;
+            break;
+          }
+          bn = b + 1 < nb ? b + 1 : 0;
+          tp_block4p<NTH, TILE>(sb, ra, fb + bn, OPS + bn * kTileStrideFp32, wave, sa, ra, st);
+          if (sb.sync) __syncthreads();
+          if (++b == nb) break;
         }
-        tp_step<NTH, TILE>(fb, b, nb, OPS, wave, sb, rb, sa, ra);
-        if (++b == nb) break;
+      } else {
+        for (int b = 0;;) {
+          tp_step<NTH, TILE>(fb, b, nb, OPS, wave, sa, ra, sb, rb);
+          if (++b == nb) {
+            sa = sb;
+            ra = rb;
+            break;
+          }
+          tp_step<NTH, TILE>(fb, b, nb, OPS, wave, sb, rb, sa, ra);
+          if (++b == nb) break;
+        }
       }
     }
 
@@ -684,14 +819,15 @@ static int grow_dev(void **p, size_t *cap, size_t need) {
   return 0;
 }
 
-template <int NTH, bool SPLIT>
+template <int NTH, bool SPLIT, bool K4P>
 static int launch_tp(qcs_hip_state *s, const TilePass *dpass, int pi,
                      const TileBlock *dblocks, uint64_t ntiles, int nblocks) {
   // once per kernel instance (states may launch from separate threads)
   static std::once_flag once;
   static int attr_rc = 0;
   std::call_once(once, [] {
-    const void *fn = reinterpret_cast<const void *>(&k_tile_program<kTileBitsFixed, NTH, SPLIT>);
+    const void *fn =
+        reinterpret_cast<const void *>(&k_tile_program<kTileBitsFixed, NTH, SPLIT, K4P>);
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        int(kLdsPerCu));
     if (e != hipSuccess) {
@@ -715,7 +851,7 @@ static int launch_tp(qcs_hip_state *s, const TilePass *dpass, int pi,
   const uint64_t per_cu = kLdsPerCu / lds;  // work-groups that fit per CU
   uint64_t grid = uint64_t(s->tile_grid) * (per_cu > 1 ? 2 : 1);
   if (grid > ntiles) grid = ntiles;
-  hipLaunchKernelGGL((k_tile_program<kTileBitsFixed, NTH, SPLIT>), dim3(unsigned(grid)),
+  hipLaunchKernelGGL((k_tile_program<kTileBitsFixed, NTH, SPLIT, K4P>), dim3(unsigned(grid)),
                      dim3(NTH), lds, s->stream, static_cast<float2 *>(s->vec),
                      dpass, pi, dblocks, s->wbuf, ntiles);
   return launch_check("k_tile_program");
@@ -754,7 +890,13 @@ static int pick_lanes(int avail, int lanes[5]) {
 // target), lane bit 4 (the half a column falls in) on tb0 where it can, the
 // rest the lowest positions left.  Returns the half-wave's conflict factor,
 // 2^(5 - rank of the bank bits of lane bits 0..3 and tb0).
-static int pick_lanes16(int avail, int tb0, int lanes[5]) {
+static int pick_lanes16(int avail, int tb0, int lanes[5], int half = -1) {
+  if (half >= 0 && ((avail >> half) & 1)) {
+    // lane bit 4 on `half`, lane bits 0..3 from the other positions
+    const int f = pick_lanes16(avail & ~(1 << half), tb0, lanes);
+    lanes[4] = half;
+    return f;
+  }
   int n = 0, used = 0;
   for (int c = 0; c < 5 && n < 4; ++c) {
     if (c == tb0) continue;
@@ -802,7 +944,8 @@ static int scatter_off(int v, const int *pos, int npos) {
 // tables.  Lane bits are conflict-free where the remaining positions allow,
 // else the lowest ones; form16: the block runs in the 16-column form.
 static void make_tile_block(const qcs_hip_pass &ps, const qcs_hip_block &blk, int wslot,
-                            int wmask, int nwb, bool sync, bool form16, TileBlock *tb) {
+                            int wmask, int nwb, bool sync, bool form16, int half,
+                            TileBlock *tb) {
   std::memset(tb, 0, sizeof(TileBlock));
   const int k = blk.k, R = 1 << k;
   tb->k = k;
@@ -817,7 +960,7 @@ static void make_tile_block(const qcs_hip_pass &ps, const qcs_hip_block &blk, in
     if ((wmask >> p) & 1) wpos[nw++] = p;
   int avail = ((1 << kTileBitsFixed) - 1) & ~tmask & ~wmask;
   if (form16)
-    (void)pick_lanes16(avail, bank_bit(tpos[0]), lanes);
+    (void)pick_lanes16(avail, bank_bit(tpos[0]), lanes, half);
   else
     (void)pick_lanes(avail, lanes);
   for (int i = 0; i < 5; ++i) avail &= ~(1 << lanes[i]);
@@ -838,7 +981,7 @@ static void make_tile_block(const qcs_hip_pass &ps, const qcs_hip_block &blk, in
 // kernel a barrier costs more than that).  With FP32 math, blocks of k <= 3
 // run in the 16-column form, whose conflicts also depend on the first target.
 static void layout_pass(const qcs_hip_pass &ps, const qcs_hip_block *blocks, int nwb,
-                        bool split, TileBlock *tb) {
+                        bool split, bool k4p, TileBlock *tb) {
   // conflict factor of every set of free positions (and, for the 16-column
   // form, bank bit of the first target), and the candidate wave bits of each
   // width, computed once per process
@@ -857,7 +1000,7 @@ static void layout_pass(const qcs_hip_pass &ps, const qcs_hip_block *blocks, int
       if (__builtin_popcount(unsigned(m)) <= 4) masks[__builtin_popcount(unsigned(m))].push_back(m);
   });
   const int end = ps.first_block + ps.num_blocks;
-  auto form16 = [&](int i) { return !split && blocks[i].k <= 3; };
+  auto form16 = [&](int i) { return !split && (blocks[i].k <= 3 || k4p); };
   std::vector<int> tm(size_t(ps.num_blocks));
   for (int i = 0; i < ps.num_blocks; ++i) tm[i] = target_mask(ps, blocks[ps.first_block + i]);
   for (int b = ps.first_block; b < end;) {
@@ -880,8 +1023,36 @@ static void layout_pass(const qcs_hip_pass &ps, const qcs_hip_block *blocks, int
         best_cost = cost;
       }
     }
+    // pipelined k = 4 blocks: one half bit (lane bit 4) for the whole run
+    // where a position is free in all its blocks, the one with the fewest
+    // bank conflicts, so consecutive blocks can read ahead
+    int half = -1;
+    if (k4p) {
+      int best_h = 0;
+      const int freepos = (kSets - 1) & ~u & ~best;
+      for (int h = 0; h < kTileBitsFixed; ++h) {
+        if (!((freepos >> h) & 1)) continue;
+        int cost = 0, lanes[5];
+        for (int i = b; i < e; ++i) {
+          const int t = tm[i - ps.first_block], a = (kSets - 1) & ~t & ~best;
+          cost += form16(i) ? pick_lanes16(a, bank_bit(__builtin_ctz(unsigned(t))), lanes, h)
+                            : factor[a];
+        }
+        if (half < 0 || cost < best_h) {
+          half = h;
+          best_h = cost;
+        }
+      }
+    }
     for (int i = b; i < e; ++i)
-      make_tile_block(ps, blocks[i], i, best, nwb, i == e - 1, form16(i), &tb[i]);
+      make_tile_block(ps, blocks[i], i, best, nwb, i == e - 1, form16(i), half, &tb[i]);
+    // a k = 4 block reads the next block's first half ahead when that block is
+    // also k = 4, in this run, with the same half bit
+    for (int i = b; i + 1 < e; ++i)
+      tb[i].early = (k4p && blocks[i].k == 4 && blocks[i + 1].k == 4 &&
+                     tb[i].lanecol[16] == tb[i + 1].lanecol[16])
+                        ? 1
+                        : 0;
     b = e;
   }
 }
@@ -895,19 +1066,50 @@ int tiles_apply(qcs_hip_state *s, const qcs_hip_pass *passes, int np,
   int rc;
   if ((rc = grow_dev(&s->tpass_dev, &s->tpass_cap, pbytes))) return rc;
   if ((rc = grow_dev(&s->tblock_dev, &s->tblock_cap, tbytes))) return rc;
+  const int nwb = s->tile_threads == 1024 ? 4 : 3;  // wave-index bits
+  const bool split_math = s->tile_math == QCS_HIP_TILE_MATH_SPLIT_BF16;
+  // k = 4 blocks as pipelined 16-column halves: FP32 math, 1024 threads;
+  // that kernel runs every block as a k = 4 block, so smaller blocks get
+  // more targets (their unitary composes to U x I on them), taken from the
+  // neighbouring blocks' targets where possible so no run needs more bits
+  const bool k4p = !split_math && s->tile_threads == 1024 &&
+                   s->tile_k4 == QCS_HIP_TILE_K4_HALVES;
+  std::vector<qcs_hip_block> padded;
+  const qcs_hip_block *lay = blocks;
+  if (k4p) {
+    padded.assign(blocks, blocks + nb);
+    for (int p = 0; p < np; ++p) {
+      const int b0 = passes[p].first_block, b1 = b0 + passes[p].num_blocks;
+      for (int b = b0; b < b1; ++b) {
+        qcs_hip_block &bl = padded[size_t(b)];
+        if (bl.k >= QCS_HIP_TILE_KMAX) continue;
+        int cand[3 * QCS_HIP_TILE_KMAX + QCS_HIP_TILE_BITS], nc = 0;
+        for (int nbh : {b - 1, b + 1})
+          if (nbh >= b0 && nbh < b1)
+            for (int q = 0; q < blocks[nbh].k; ++q) cand[nc++] = blocks[nbh].targets[q];
+        for (int j = passes[p].tile_bits - 1; j >= 0; --j) cand[nc++] = passes[p].bits[j];
+        for (int c = 0; c < nc && bl.k < QCS_HIP_TILE_KMAX; ++c) {
+          bool have = false;
+          for (int q = 0; q < bl.k; ++q) have = have || bl.targets[q] == cand[c];
+          if (!have) bl.targets[bl.k++] = cand[c];
+        }
+        std::sort(bl.targets, bl.targets + bl.k);
+      }
+    }
+    lay = padded.data();
+  }
   char *stage = nullptr;
-  if ((rc = compose_blocks(s, blocks, nb, gates, ng, pbytes + tbytes, &stage)))
+  if ((rc = compose_blocks(s, lay, nb, gates, ng, pbytes + tbytes, &stage)))
     return rc;
   TilePass *tp = reinterpret_cast<TilePass *>(stage);
   TileBlock *tb = reinterpret_cast<TileBlock *>(stage + pbytes);
-  const int nwb = s->tile_threads == 1024 ? 4 : 3;  // wave-index bits
   for (int p = 0; p < np; ++p) {
     std::memset(&tp[p], 0, sizeof(TilePass));
     tp[p].tb = passes[p].tile_bits;
     tp[p].nblocks = passes[p].num_blocks;
     tp[p].first_block = passes[p].first_block;
     for (int j = 0; j < passes[p].tile_bits; ++j) tp[p].sbits[j] = passes[p].bits[j];
-    layout_pass(passes[p], blocks, nwb, s->tile_math == QCS_HIP_TILE_MATH_SPLIT_BF16, tb);
+    layout_pass(passes[p], lay, nwb, split_math, k4p, tb);
   }
   QCS_HIP_TRY(hipMemcpyAsync(s->tpass_dev, tp, pbytes, hipMemcpyHostToDevice,
                              s->stream));
@@ -930,11 +1132,12 @@ int tiles_apply(qcs_hip_state *s, const qcs_hip_pass *passes, int np,
     timing_begin(s, QCS_HIP_K_FUSED, 2.0 * N * double(s->amp_bytes), flops, &tl);
     const int nbp = passes[p].num_blocks;
     if (s->tile_threads == 1024)
-      rc = split ? launch_tp<1024, true>(s, dp, p, db, ntiles, nbp)
-                 : launch_tp<1024, false>(s, dp, p, db, ntiles, nbp);
+      rc = split ? launch_tp<1024, true, false>(s, dp, p, db, ntiles, nbp)
+           : k4p ? launch_tp<1024, false, true>(s, dp, p, db, ntiles, nbp)
+                 : launch_tp<1024, false, false>(s, dp, p, db, ntiles, nbp);
     else
-      rc = split ? launch_tp<512, true>(s, dp, p, db, ntiles, nbp)
-                 : launch_tp<512, false>(s, dp, p, db, ntiles, nbp);
+      rc = split ? launch_tp<512, true, false>(s, dp, p, db, ntiles, nbp)
+                 : launch_tp<512, false, false>(s, dp, p, db, ntiles, nbp);
     if (rc) return rc;
     if ((rc = timing_end(s, &tl))) return rc;
   }

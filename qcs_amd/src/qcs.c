@@ -459,10 +459,31 @@ int qc_measure(t_q_circuit *circuit, int qubit) {
   return outcome;
 }
 
+/* src/qcs.c:296-302: qc_measure of every qubit in order.  The n uniforms
+ * are drawn first, in the same order, and the measurements run on the
+ * device one after another with a single synchronisation at the end. */
 void qc_measure_all(t_q_circuit *circuit, int *results) {
-  int i;
+  int i, n;
+  int *qubits;
+  double *u;
   if (circuit == NULL || results == NULL) return;
-  for (i = 0; i < circuit->num_qubits; i++) results[i] = qc_measure(circuit, i);
+  n = circuit->num_qubits;
+  qubits = (int *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int));
+  u = (double *)malloc((size_t)(n > 0 ? n : 1) * sizeof(double));
+  if (circuit->state == NULL || n <= 0 || !qubits || !u) {
+    for (i = 0; i < n; i++) results[i] = qc_measure(circuit, i);
+  } else {
+    flush_pending(circuit);
+    for (i = 0; i < n; i++) {
+      qubits[i] = i;
+      u[i] = rand() / (double)RAND_MAX;
+    }
+    if (check(circuit, qcs_hip_measure_seq(circuit->state, n, qubits, u, results),
+              "measure"))
+      for (i = 0; i < n; i++) results[i] = 0;
+  }
+  free(qubits);
+  free(u);
   add_history(circuit, "MEASURE", -1, -1, 0.0);
 }
 

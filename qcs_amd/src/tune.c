@@ -1,13 +1,17 @@
 /*
  * tune.c -- autotuner of the fused and batched forms, with a result cache.
  *
- * Fused form (qc_autotune): for every fusion width k the one-block-per-pass
- * kernel has launch variants (tile size, threads, pipelining); each is timed
- * on the circuit's own state with identity blocks (an identity block leaves
- * every amplitude unchanged: products by 1 and by 0 are exact), the fastest
- * is kept per (device, n, k), and the fusion width for the recorded gates is
- * the k that minimises (blocks the planner makes at width k) x (time of one
- * pass at width k).
+ * Fused form (qc_autotune), one pass per block: for every fusion width k
+ * the one-block-per-pass kernel has launch variants (tile size, threads,
+ * pipelining); each is timed on the circuit's own state with identity blocks
+ * (an identity block leaves every amplitude unchanged: products by 1 and by
+ * 0 are exact), the fastest is kept per (device, n, k), and the fusion width
+ * for the recorded gates is the k that minimises (blocks the planner makes
+ * at width k) x (time of one pass at width k).
+ *
+ * Fused form as tile programs (QCS_FUSE_TILES, QCS_FUSE_SPLIT_BF16): the
+ * circuit's own pass plan is timed, with identity blocks, at each fusion
+ * width, work-group size and grid; the cost is the sum of its pass times.
  *
  * Batched form (qc_batch_autotune): for every (precision, n) whose states
  * are streamed through LDS, candidate tile sizes (and fusion widths for
@@ -21,6 +25,7 @@
  * $XDG_CACHE_HOME/qcs_amd/tune.txt, else $HOME/.cache/qcs_amd/tune.txt.
  * Lines are
  *   <device> fused <n> <k> <variant> <microseconds per pass>
+ *   <device> tiles[split] <n> <structure hash> <k> <threads> <grid> <k4 form> <ms>
  *   <device> batch <precision> <n> <tile bits> <k> <milliseconds>
  *   <device> batchthreads <threads> <milliseconds>
  * and the last matching line wins.
@@ -138,6 +143,122 @@ static int time_variant(qcs_hip_state *s, int n, int k, int variant,
   return 0;
 }
 
+/* ---------------------------------------------------------------------- */
+/* tile programs                                                            */
+/* ---------------------------------------------------------------------- */
+
+/* FNV-1a over the kind and qubits of the recorded gates (not the angles):
+ * circuits with the same structure get the same pass plans */
+static unsigned long structure_hash(const t_q_circuit *c) {
+  unsigned long h = 1469598103934665603UL;
+  int i, j;
+  for (i = 0; i < c->pending_size; i++) {
+    int v[3];
+    v[0] = c->pending[i].kind;
+    v[1] = c->pending[i].target;
+    v[2] = c->pending[i].control;
+    for (j = 0; j < 3; j++) {
+      h ^= (unsigned long)(unsigned)v[j];
+      h *= 1099511628211UL;
+    }
+  }
+  return h;
+}
+
+/* device time (ms) of the circuit's own tile-program plan at fusion width k
+ * under the state's current launch shape: the sum over its passes, every
+ * block composed from no gates (the identity, so the state is unchanged) */
+static int time_tile_plan(t_q_circuit *c, int k, int max_blocks, double *ms_out) {
+  qcs_hip_block *blocks = NULL;
+  qcs_hip_gate *ordered = NULL;
+  qcs_hip_pass *passes = NULL;
+  int nb = 0, np = 0, b, rc;
+  double ms = 0.0, bytes = 0.0, flops = 0.0;
+  long launches = 0;
+  if (qcs_pass_plan(c->pending, c->pending_size, c->num_qubits, QCS_HIP_TILE_BITS,
+                    QCS_HIP_TILE_LOW_BITS, k, max_blocks, &passes, &np, &blocks, &nb,
+                    &ordered) != 0)
+    return QCS_HIP_ENOMEM;
+  for (b = 0; b < nb; b++) {
+    blocks[b].first_gate = 0;
+    blocks[b].num_gates = 0;
+  }
+  rc = qcs_hip_timing_enable(c->state, 1);
+  if (!rc) rc = qcs_hip_apply_passes(c->state, passes, np, blocks, nb, ordered, c->pending_size);
+  if (!rc)
+    rc = qcs_hip_timing_read(c->state, QCS_HIP_K_FUSED, &ms, &launches, &bytes, &flops);
+  qcs_hip_timing_enable(c->state, 0);
+  free(passes);
+  free(blocks);
+  free(ordered);
+  *ms_out = ms;
+  return rc;
+}
+
+/* Tile-program circuits: fusion width k in {3, 4}, work-group size 512 or
+ * 1024, a persistent grid of one or two work-groups per CU and (FP32 math on
+ * 1024 threads) the form of k = 4 blocks, timed on the
+ * circuit's own pass plan (the sum of its pass times, not blocks x a block's
+ * time: a pass holds several blocks, and what a block costs depends on its
+ * width and on the pass around it).  Cached per (device, n, arithmetic,
+ * gate structure). */
+static int set_tile_shape(qcs_hip_state *s, int threads, int grid, int form) {
+  int rc;
+  if ((rc = qcs_hip_set_tuning(s, QCS_HIP_TUNE_TILE_THREADS, threads)) ||
+      (rc = qcs_hip_set_tuning(s, QCS_HIP_TUNE_TILE_GRID, grid)) ||
+      (rc = qcs_hip_set_tuning(s, QCS_HIP_TUNE_TILE_K4_FORM, form)))
+    return rc;
+  return 0;
+}
+
+static int autotune_tiles(t_q_circuit *c, const char *dev) {
+  char key[200], rest[TUNE_LINE], line[TUNE_LINE];
+  const int split = (c->options & QCS_FUSE_SPLIT_BF16) != 0;
+  const int max_blocks = split ? QCS_HIP_TILE_MAX_BLOCKS_SPLIT : QCS_HIP_TILE_MAX_BLOCKS;
+  const char *colon = strchr(dev, ':');
+  const int cus = colon ? atoi(colon + 1) : 256;
+  int k, th, g, form, best_k = QCS_HIP_TILE_KMAX, best_th = 1024, best_grid = cus;
+  int best_form = QCS_HIP_TILE_K4_CHAIN, rc;
+  double best = 1e30, ms;
+  if (c->pending_size == 0) {
+    c->tuned_k = best_k;
+    return best_k;
+  }
+  sprintf(key, "%s tiles%s %d %lx", dev, split ? "split" : "", c->num_qubits,
+          structure_hash(c));
+  if (cache_find(key, rest, sizeof(rest)) &&
+      sscanf(rest, "%d %d %d %d", &best_k, &best_th, &best_grid, &best_form) == 4 &&
+      best_k >= 1 && best_k <= QCS_HIP_TILE_KMAX &&
+      set_tile_shape(c->state, best_th, best_grid, best_form) == 0) {
+    c->tuned_k = best_k;
+    return best_k;
+  }
+  /* one untimed run loads the kernels and warms the caches */
+  if ((rc = time_tile_plan(c, QCS_HIP_TILE_KMAX, max_blocks, &ms))) return rc;
+  for (th = 512; th <= 1024; th *= 2)
+    for (g = 1; g <= 2; g++)
+      for (k = 3; k <= QCS_HIP_TILE_KMAX; k++)
+        /* the pipelined k = 4 form exists for FP32 math on 1024 threads */
+        for (form = QCS_HIP_TILE_K4_CHAIN;
+             form <= ((th == 1024 && !split) ? QCS_HIP_TILE_K4_HALVES : QCS_HIP_TILE_K4_CHAIN);
+             form++) {
+          if ((rc = set_tile_shape(c->state, th, g * cus, form))) return rc;
+          if ((rc = time_tile_plan(c, k, max_blocks, &ms))) return rc;
+          if (ms < best) {
+            best = ms;
+            best_k = k;
+            best_th = th;
+            best_grid = g * cus;
+            best_form = form;
+          }
+        }
+  if ((rc = set_tile_shape(c->state, best_th, best_grid, best_form))) return rc;
+  sprintf(line, "%s %d %d %d %d %.3f\n", key, best_k, best_th, best_grid, best_form, best);
+  cache_append(line);
+  c->tuned_k = best_k;
+  return best_k;
+}
+
 int qc_autotune(t_q_circuit *c) {
   char dev[96], key[160], rest[TUNE_LINE];
   double t_pass[QCS_HIP_MAX_FUSE + 1];
@@ -145,6 +266,11 @@ int qc_autotune(t_q_circuit *c) {
   double best_cost = 0.0;
   if (!c || !c->state || !(c->options & QCS_FUSED_FP32)) return QCS_HIP_EINVAL;
   n = c->num_qubits;
+  if ((c->options & (QCS_FUSE_TILES | QCS_FUSE_SPLIT_BF16)) &&
+      n >= QCS_HIP_TILE_BITS + 3) {
+    if ((rc = device_key(dev, sizeof(dev)))) return rc;
+    return autotune_tiles(c, dev);
+  }
   kmax = n < QCS_HIP_MAX_FUSE ? n : QCS_HIP_MAX_FUSE;
   if ((c->options & (QCS_FUSE_TILES | QCS_FUSE_SPLIT_BF16)) && kmax > QCS_HIP_TILE_KMAX)
     kmax = QCS_HIP_TILE_KMAX;

@@ -168,6 +168,54 @@ def test_tile_programs_equal_single_block_passes(qlib, n, threads):
     assert 1 - abs(np.vdot(ref, got)) ** 2 / np.vdot(got, got).real <= 1e-6
 
 
+@pytest.mark.parametrize("n,seed", [(16, 1), (19, 2), (22, 3)])
+def test_tile_k4_halves_form(qlib, n, seed):
+    """The pipelined form of k = 4 blocks (two 16-column halves on the 16x16x4
+    instruction; smaller blocks padded to four targets; a block's second half
+    stored behind the next block's first half, whose B operands are read
+    ahead) applies the same blocks as the 32x32 chain: the states agree to
+    FP32 rounding and stay within the form's 1 - F of the FP64 simulation.
+    A block read ahead of what it depends on, a half stored to the wrong
+    columns or a block padded on a wrong target would differ by O(1)."""
+    from test_fusion_plan import pass_plan
+    hip, _ = qlib
+    print(f"[fused] k4 halves n={n}", flush=True)
+    kinds = [C.H, C.X, C.Y, C.Z, C.P, C.RX, C.RY, C.RZ, C.CNOT, C.CPHASE, C.SWAP]
+    ops = C.random_circuit(n, 400, 777 * seed + n, kinds=kinds)
+    for o in (ops, C.layered(n, 6, seed)):
+        passes, blocks, ordered = pass_plan(o, n)
+        assert any(b.k < 4 for b in blocks) and any(b.k == 4 for b in blocks)
+        P = (qcs_amd.HipPass * len(passes))(*passes)
+        B = (qcs_amd.HipBlock * len(blocks))(*blocks)
+        G = (qcs_amd.HipGate * len(ordered))(*ordered)
+        psi = rand_state(n, np.random.default_rng(n + seed))
+        with qcs_amd.Circuit(n, qcs_amd.PLAIN_FP32) as a, \
+                qcs_amd.Circuit(n, qcs_amd.PLAIN_FP32) as b:
+            for st, form in ((a, qcs_amd.TILE_K4_CHAIN), (b, qcs_amd.TILE_K4_HALVES)):
+                st.set_amplitudes(psi)
+                assert hip.qcs_hip_set_tuning(st.device_state, qcs_amd.TUNE_TILE_K4_FORM,
+                                              form) == 0
+                rc = hip.qcs_hip_apply_passes(st.device_state, P, len(passes), B,
+                                              len(blocks), G, len(ordered))
+                assert rc == 0, hip.qcs_hip_last_error()
+            chain, halves = a.amplitudes(), b.amplitudes()
+        assert np.linalg.norm(halves - chain) <= 1e-5
+        ref = HP.simulate(n, o, psi.astype(np.complex64).astype(complex))
+        assert 1 - abs(np.vdot(ref, halves)) ** 2 / np.vdot(halves, halves).real <= 1e-6
+    # through the qc_* API: a tile-program circuit with the form set
+    ops = C.layered(n, 8, seed + 10)
+    with qcs_amd.Circuit(n) as ref, \
+            qcs_amd.Circuit(n, qcs_amd.FUSED_FP32 | qcs_amd.FUSE_TILES) as f:
+        assert hip.qcs_hip_set_tuning(f.device_state, qcs_amd.TUNE_TILE_K4_FORM,
+                                      qcs_amd.TILE_K4_HALVES) == 0
+        ref.apply(ops)
+        f.apply(ops)
+        omf, mx = f.compare_with(ref)
+        assert omf <= 1e-6 and mx < 1e-5, (omf, mx)
+    with qcs_amd.Circuit(n, qcs_amd.PLAIN_FP32) as a:
+        assert hip.qcs_hip_set_tuning(a.device_state, qcs_amd.TUNE_TILE_K4_FORM, 2) == -1
+
+
 def test_apply_passes_rejects_bad_plans(qlib):
     from test_fusion_plan import pass_plan
     hip, _ = qlib

@@ -254,3 +254,26 @@ def test_plain_fp32_close_to_fp64(qlib):
     with gpu_run(n, ops) as a, gpu_run(n, ops, qcs_amd.PLAIN_FP32) as b:
         omf, mx = a.compare_with(b)
         assert omf < 1e-9 and mx < 1e-5
+
+
+@pytest.mark.parametrize("n,opt", [(4, qcs_amd.PLAIN_FP64), (9, qcs_amd.PLAIN_FP64),
+                                   (14, qcs_amd.PLAIN_FP32), (20, qcs_amd.PLAIN_FP64),
+                                   (11, qcs_amd.REFERENCE_COMPAT)])
+def test_measure_all_on_device_equals_measure_loop(qlib, cl, n, opt):
+    # qc_measure_all decides every outcome on the device (one host round
+    # trip); it must give what qc_measure of each qubit in turn gives from
+    # the same rand() stream, outcomes and final state
+    ops = C.random_circuit(n, 8 * n, 555 + n, kinds=ALL_KINDS)
+    with gpu_run(n, ops, opt) as a, gpu_run(n, ops, opt) as b:
+        for seed in (1, 2, 3):
+            cl.srand(seed)
+            got = a.measure_all_list()
+            cl.srand(seed)
+            want = [b.measure(q) for q in range(n)]
+            assert got == want, (seed, got, want)
+            np.testing.assert_allclose(a.amplitudes(), b.amplitudes(), rtol=0, atol=1e-12)
+            if opt != qcs_amd.REFERENCE_COMPAT:  # compat gates need not keep the norm
+                assert a.get_probability(int("".join(map(str, got[::-1])), 2)) > 0.999
+            # a fresh superposition for the next seed
+            a.apply(ops)
+            b.apply(ops)

@@ -77,3 +77,44 @@ def test_batch_autotune_keeps_results(qlib, cl, cache):
     nlines = len(cache.read_text().splitlines())
     run(True)
     assert len(cache.read_text().splitlines()) == nlines
+
+
+def tuning(hip, state, knob):
+    import ctypes
+    v = ctypes.c_int()
+    assert hip.qcs_hip_get_tuning(state, knob, ctypes.byref(v)) == 0
+    return v.value
+
+
+def test_tile_autotune_times_the_pass_plan_and_caches(qlib, cache):
+    # tile programs: fusion width, work-group size and grid are timed on the
+    # circuit's own pass plan (identity blocks) and cached per gate structure
+    hip, _ = qlib
+    n = 18
+    opts = qcs_amd.FUSED_FP32 | qcs_amd.FUSE_TILES
+    with qcs_amd.Circuit(n, opts) as c, qcs_amd.Circuit(n) as exact:
+        ops = C.layered(n, 6, 12)
+        c.apply(ops)
+        k = c.autotune()
+        assert k in (3, 4)
+        lines = cache.read_text().splitlines()
+        assert len(lines) == 1 and f" tiles {n} " in lines[0]
+        _, _, _, hsh, ck, cth, cgrid, cform, ms = lines[0].split()
+        assert int(ck) == k and float(ms) > 0
+        st = c.device_state  # flushes with the tuned plan
+        assert tuning(hip, st, qcs_amd.TUNE_TILE_THREADS) == int(cth) in (512, 1024)
+        assert tuning(hip, st, qcs_amd.TUNE_TILE_GRID) == int(cgrid)
+        assert tuning(hip, st, qcs_amd.TUNE_TILE_K4_FORM) == int(cform) in (0, 1)
+        exact.apply(ops)
+        omf, _ = c.compare_with(exact)
+        assert omf <= 1e-6
+        assert c.get_num_fused_passes() < c.get_num_fused_blocks()
+    # the same structure with other angles: answered from the cache
+    with qcs_amd.Circuit(n, opts) as c2, qcs_amd.Circuit(n) as exact:
+        ops = C.layered(n, 6, 99)
+        c2.apply(ops)
+        assert c2.autotune() == k
+        assert len(cache.read_text().splitlines()) == 1
+        assert tuning(hip, c2.device_state, qcs_amd.TUNE_TILE_THREADS) == int(cth)
+        exact.apply(ops)
+        assert c2.compare_with(exact)[0] <= 1e-6
