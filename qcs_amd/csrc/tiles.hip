@@ -221,6 +221,16 @@ __device__ __forceinline__ void load_regs(BlockRegs &r, int blk) {
   r.lp2 = lds_load<int>(blk + kTileAFp32 + 4 * (l32 ^ 16));
 }
 
+// The pipelined k = 4 form reads the lane's column in each half directly:
+// lp = lane table entry (lane & 15), lp2 = entry (lane & 15) | 16.
+__device__ __forceinline__ void load_regs4p(BlockRegs &r, int blk) {
+  const int lane = threadIdx.x & 63, c = lane & 15;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) r.a[v] = lds_load<f4v>(blk + (v * 64 + lane) * 16);
+  r.lp = lds_load<int>(blk + kTileAFp32 + 4 * c);
+  r.lp2 = lds_load<int>(blk + kTileAFp32 + 4 * (c | 16));
+}
+
 // What every block kernel needs of its block: uniform row offsets and the
 // wave's column part (scalar registers), the lane's column part and output
 // offsets.
@@ -400,14 +410,16 @@ __device__ __forceinline__ void tp_block16(const TileBlock *__restrict__ bl,
 // half-0 chain, when that block is a k = 4 block of the same run with the
 // same half bit (lane bit 4): its first half then reads only amplitudes that
 // this block's first half wrote, or that no block of the run wrote, so its
-// B reads go out before this block's second half is stored.  Output rows of
+// B reads go out before this block's second half is stored.  Otherwise the
+// half-1 results are stored (and, at the end of a run, the work-group
+// waits) before the next block's B reads.  Output rows of
 // row block rb are those of tp_block16 with target bit 3 = rb; the A operands
 // of instruction sp of row block rb sit at index 8 rb + sp.
 struct K4State {
   float bv0[8];  // first-half B operands of the current block, read ahead
   f4v acc[2];    // second-half results of the previous block, not yet stored
   int po, pr1, pr3;  // their base address, row offsets of target bits 1, 3
-  int have_b0, pend;
+  int pend;
 };
 
 // One column half's two 16x16 accumulator tiles (row blocks 0 and 1) to LDS.
@@ -428,27 +440,37 @@ __device__ __forceinline__ void store_k4_half(int o, int r1, int r3, const f4v &
   }
 }
 
+// First-half B operands of a pipelined k = 4 block into st.bv0.
+// Lane (q, c) of a 16x16x4 instruction: q & 1 selects the row of target
+// bit 0, q >> 1 the plane (B) or the row of target bit 2 (results); as
+// all-ones masks, so a row offset is selected with one AND.
+__device__ __forceinline__ int q1_mask() { return -int((threadIdx.x >> 4) & 1); }
+__device__ __forceinline__ int q2_mask() { return -int((threadIdx.x >> 5) & 1); }
+
+// First-half B operands of a pipelined k = 4 block into st.bv0.
+__device__ __forceinline__ void read_k4_b0(const BlockScalars &d, const BlockRegs &r,
+                                           K4State &st) {
+  const int bq = (d.rb[0] & q1_mask()) ^ (kTilePlane & q2_mask()) ^ d.up;
+#pragma unroll
+  for (int sp = 0; sp < 8; ++sp) st.bv0[sp] = lds_load<float>(r.lp ^ bq ^ d.ro(2 * sp));
+}
+
 template <int NTH, int TILE>
 __device__ __forceinline__ void tp_block4p(const BlockScalars &d, const BlockRegs &r,
                                            const TileBlock *__restrict__ nbl, int nblk,
                                            int wave, BlockScalars &nd, BlockRegs &nr,
                                            K4State &st) {
   static_assert((TILE >> 4) / 32 / (NTH / 64) == 1, "one 32-column group per wave");
-  const int lane = threadIdx.x & 63, q = lane >> 4;
   float wa[16];
 #pragma unroll
   for (int s = 0; s < 16; ++s) wa[s] = r.a[s >> 2][s & 3];
   int ro2[8];
 #pragma unroll
   for (int sp = 0; sp < 8; ++sp) ro2[sp] = d.ro(2 * sp);
-  const int lc0 = (lane & 16) ? r.lp2 : r.lp, lc1 = (lane & 16) ? r.lp : r.lp2;
-  const int rq = (q & 1) ? d.rb[0] : 0;
-  const int bq = rq ^ ((q >> 1) ? kTilePlane : 0) ^ d.up;  // B: row 2sp + (q & 1), part q >> 1
-  const int oq = rq ^ ((q >> 1) ? d.rb[2] : 0) ^ d.up;     // results: rows (q & 1) | (q >> 1) << 2
-  if (!st.have_b0) {
-#pragma unroll
-    for (int sp = 0; sp < 8; ++sp) st.bv0[sp] = lds_load<float>(lc0 ^ bq ^ ro2[sp]);
-  }
+  const int lc0 = r.lp, lc1 = r.lp2;  // the lane's column in each half
+  const int rq = d.rb[0] & q1_mask();
+  const int bq = rq ^ (kTilePlane & q2_mask()) ^ d.up;  // B: row 2sp + (q & 1), part q >> 1
+  const int oq = rq ^ (d.rb[2] & q2_mask()) ^ d.up;     // results: rows (q & 1) | (q >> 1) << 2
   __builtin_amdgcn_sched_barrier(0);
   const f4v z = {0.f, 0.f, 0.f, 0.f};
   f4v a00 = z, a01 = z;
@@ -464,32 +486,32 @@ __device__ __forceinline__ void tp_block4p(const BlockScalars &d, const BlockReg
   for (int sp = 0; sp < 8; ++sp) bv1[sp] = lds_load<float>(lc1 ^ bq ^ ro2[sp]);
   nd = load_scalars(nbl, wave);
   __builtin_amdgcn_sched_barrier(0);
-  f4v a10 = z, a11 = z;
+  // the second half accumulates straight into the pending registers (the
+  // previous block's are stored above)
+  st.acc[0] = z;
+  st.acc[1] = z;
 #pragma unroll
   for (int sp = 0; sp < 8; ++sp) {
-    a10 = __builtin_amdgcn_mfma_f32_16x16x4f32(wa[sp], bv1[sp], a10, 0, 0, 0);
-    a11 = __builtin_amdgcn_mfma_f32_16x16x4f32(wa[8 + sp], bv1[sp], a11, 0, 0, 0);
+    st.acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(wa[sp], bv1[sp], st.acc[0], 0, 0, 0);
+    st.acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(wa[8 + sp], bv1[sp], st.acc[1], 0, 0, 0);
   }
   __builtin_amdgcn_sched_barrier(0);
-  load_regs(nr, nblk);
+  load_regs4p(nr, nblk);
   store_k4_half(lc0 ^ oq, d.rb[1], d.rb[3], a00, a01);
-  if (d.early) {
-    const int nlc0 = (lane & 16) ? nr.lp2 : nr.lp;
-    const int nbq = ((q & 1) ? nd.rb[0] : 0) ^ ((q >> 1) ? kTilePlane : 0) ^ nd.up;
-#pragma unroll
-    for (int sp = 0; sp < 8; ++sp) st.bv0[sp] = lds_load<float>(nlc0 ^ nbq ^ nd.ro(2 * sp));
-    st.have_b0 = 1;
-    st.acc[0] = a10;
-    st.acc[1] = a11;
-    st.po = lc1 ^ oq;
-    st.pr1 = d.rb[1];
-    st.pr3 = d.rb[3];
-    st.pend = 1;
-  } else {
-    store_k4_half(lc1 ^ oq, d.rb[1], d.rb[3], a10, a11);
-    st.have_b0 = 0;
-    st.pend = 0;
+  st.po = lc1 ^ oq;
+  st.pr1 = d.rb[1];
+  st.pr3 = d.rb[3];
+  st.pend = d.early;
+  // Unless the next block may read ahead, this block's second half is
+  // stored now and, at the end of a run, the work-group waits; either way
+  // the next block's first-half inputs are final below.  (After a pass's
+  // last block the reads serve nothing: the next tile's first block reads
+  // again once the tile is in LDS.)
+  if (!d.early) {
+    store_k4_half(st.po, st.pr1, st.pr3, st.acc[0], st.acc[1]);
+    if (d.sync) __syncthreads();
   }
+  read_k4_b0(nd, nr, st);
 }
 
 // One FP32 block of the pass (index b), with the next block's values read
@@ -702,7 +724,10 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
   BlockRegs ra, rb;
   if (!SPLIT) {
     sa = load_scalars(&blocks[ps.first_block], wave);
-    load_regs(ra, OPS);
+    if (K4P)
+      load_regs4p(ra, OPS);
+    else
+      load_regs(ra, OPS);
   }
   if (tile < tend) {
 #pragma unroll
@@ -753,20 +778,12 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
         // of per-lane values, read in place once a block's last chain is
         // issued; the scalars alternate between two sets
         K4State st;
-        st.have_b0 = 0;
         st.pend = 0;
-        for (int b = 0;;) {
-          int bn = b + 1 < nb ? b + 1 : 0;
+        read_k4_b0(sa, ra, st);
+        for (int b = 0; b < nb; ++b) {
+          const int bn = b + 1 < nb ? b + 1 : 0;
           tp_block4p<NTH, TILE>(sa, ra, fb + bn, OPS + bn * kTileStrideFp32, wave, sb, ra, st);
-          if (sa.sync) __syncthreads();
-          if (++b == nb) {
-            sa = sb;
-            break;
-          }
-          bn = b + 1 < nb ? b + 1 : 0;
-          tp_block4p<NTH, TILE>(sb, ra, fb + bn, OPS + bn * kTileStrideFp32, wave, sa, ra, st);
-          if (sb.sync) __syncthreads();
-          if (++b == nb) break;
+          sa = sb;
         }
       } else {
         for (int b = 0;;) {

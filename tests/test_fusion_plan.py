@@ -145,11 +145,17 @@ def test_pass_plan_reproduces_circuit(n, tb, low, k, mb):
 
 
 def test_pass_plan_layered_30q_counts():
+    # the circuit is fused as a whole (144 blocks, as one pass per block at
+    # k = 4) and the blocks dealt into passes, so no block is cut at a pass
+    # boundary
+    assert len(plan(C.layered(30, 20, 2), 30, 4)[0]) == 144
     passes, blocks, _ = pass_plan(C.layered(30, 20, 2), 30)
-    assert (len(passes), len(blocks)) == (38, 189)
+    assert (len(passes), len(blocks)) == (53, 144)
     # the tile programs' default: tiles hold bits 0..3 (128-byte runs)
     passes, blocks, _ = pass_plan(C.layered(30, 20, 2), 30, low=4)
-    assert (len(passes), len(blocks)) == (26, 196)
+    assert (len(passes), len(blocks)) == (31, 144)
+    assert sum(b.k == 4 for b in blocks) == 143
+    assert max(p.num_blocks for p in passes) <= 22
 
 
 def test_circuit_gate_counts():

@@ -184,7 +184,9 @@ def test_tile_k4_halves_form(qlib, n, seed):
     ops = C.random_circuit(n, 400, 777 * seed + n, kinds=kinds)
     for o in (ops, C.layered(n, 6, seed)):
         passes, blocks, ordered = pass_plan(o, n)
-        assert any(b.k < 4 for b in blocks) and any(b.k == 4 for b in blocks)
+        assert any(b.k == 4 for b in blocks)
+        if o is ops:  # the random circuit also has smaller blocks to pad
+            assert any(b.k < 4 for b in blocks)
         P = (qcs_amd.HipPass * len(passes))(*passes)
         B = (qcs_amd.HipBlock * len(blocks))(*blocks)
         G = (qcs_amd.HipGate * len(ordered))(*ordered)
