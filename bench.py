@@ -45,6 +45,8 @@ METRIC = "amplitude updates/s at 30 qubits: fused FP32 vs plain FP64, at 1-F<=1e
 HBM_PEAK_GBS = 8000.0
 FP32_MFMA_PEAK_TFLOPS = 157.3
 BF16_MFMA_PEAK_TFLOPS = 2500.0
+# forms of 4-qubit blocks in FP32 tile programs (QCS_HIP_TILE_K4_*)
+K4_FORMS = {"chain": 0, "halves": 1, "gauss": 2}
 
 
 def parse():
@@ -61,10 +63,11 @@ def parse():
                    help="tiles: several fused blocks per HBM pass; split: tiles whose "
                         "blocks run as split-bf16 products; blocks: one pass per block")
     p.add_argument("--tile-threads", type=int, default=1024)
-    p.add_argument("--k4-form", choices=["chain", "halves"], default=None,
+    p.add_argument("--k4-form", choices=["chain", "halves", "gauss"], default=None,
                    help="form of 4-qubit blocks in FP32 tile programs (1024 threads): one "
-                        "32x32 chain per wave, or two pipelined 16-column halves; default: "
-                        "the autotuner's choice (chain with --no-tune)")
+                        "32x32 chain per wave, two pipelined 16-column halves, or those "
+                        "halves with three real products per complex product; default: "
+                        "the autotuner's choice (gauss with --no-tune)")
     p.add_argument("--no-plain", action="store_true", help="skip the plain FP64 run")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--no-alt", action="store_true",
@@ -964,8 +967,7 @@ def run_mode(hip, ops, n, mode, args, steps, warmup, tune):
     hip.qcs_hip_set_tuning(fused.device_state, qcs_amd.TUNE_TILE_THREADS, args.tile_threads)
     if args.k4_form:
         hip.qcs_hip_set_tuning(fused.device_state, qcs_amd.TUNE_TILE_K4_FORM,
-                               qcs_amd.TILE_K4_HALVES if args.k4_form == "halves"
-                               else qcs_amd.TILE_K4_CHAIN)
+                               K4_FORMS[args.k4_form])
     fuse_k = args.fuse_k or (4 if tiled else 5)
     autotuned = tune and not args.fuse_k and not args.k4_form
     if autotuned:
@@ -985,7 +987,7 @@ def run_mode(hip, ops, n, mode, args, steps, warmup, tune):
     hip.qcs_hip_get_tuning(fused.device_state, qcs_amd.TUNE_TILE_GRID, ctypes.byref(knob))
     tile_grid = knob.value
     hip.qcs_hip_get_tuning(fused.device_state, qcs_amd.TUNE_TILE_K4_FORM, ctypes.byref(knob))
-    k4_form = "halves" if knob.value == qcs_amd.TILE_K4_HALVES else "chain"
+    k4_form = {v: k for k, v in K4_FORMS.items()}[knob.value]
 
     def step():
         fused.set_basis_state(0)

@@ -244,7 +244,7 @@ int qcs_hip_apply_passes(qcs_hip_state *s, const qcs_hip_pass *passes,
                                           to 16 significant bits per block
                                           (1 - F per block ~1e-11) */
 /* QCS_HIP_TUNE_TILE_K4_FORM: how FP32 tile programs on 1024 threads run
- * blocks of 4 qubits, one of: */
+ * blocks of 4 qubits, one of (default QCS_HIP_TILE_K4_GAUSS): */
 #define QCS_HIP_TUNE_TILE_K4_FORM 3
 #define QCS_HIP_TILE_K4_CHAIN 0  /* one 32x32 matrix-core chain per wave
                                     (v_mfma_f32_32x32x2f32) */
@@ -252,6 +252,11 @@ int qcs_hip_apply_passes(qcs_hip_state *s, const qcs_hip_pass *passes,
                                     v_mfma_f32_16x16x4f32, software pipelined:
                                     a half's LDS reads and stores, and the next
                                     block's, overlap the other half's chain */
+#define QCS_HIP_TILE_K4_GAUSS 2  /* the pipelined halves with three real
+                                    products per complex product (Gauss):
+                                    Ur xr - Ui xi and (Ur + Ui)(xr + xi) -
+                                    Ur xr - Ui xi, 3/4 of the matrix-core
+                                    work of the other forms */
 /* QCS_HIP_TUNE_BLOCK_VARIANT(k): launch variant of the one-block-per-pass
  * matrix-core kernel for blocks of k qubits (1..5), one of: */
 #define QCS_HIP_TUNE_BLOCK_VARIANT(k) (8 + (k))

@@ -643,7 +643,8 @@ int qcs_hip_set_tuning(qcs_hip_state *s, int knob, int value) {
     return 0;
   }
   if (knob == QCS_HIP_TUNE_TILE_K4_FORM) {
-    if (value != QCS_HIP_TILE_K4_CHAIN && value != QCS_HIP_TILE_K4_HALVES)
+    if (value != QCS_HIP_TILE_K4_CHAIN && value != QCS_HIP_TILE_K4_HALVES &&
+        value != QCS_HIP_TILE_K4_GAUSS)
       return fail(QCS_HIP_EINVAL, "unknown k = 4 tile form %d", value);
     s->tile_k4 = value;
     return 0;

@@ -7,6 +7,7 @@
 //      - W: the real 2^(k+1) x 2^(k+1) FP32 form of U with interleaved
 //        {re, im} rows/columns: W[2i][2j] = Ur, W[2i][2j+1] = -Ui,
 //        W[2i+1][2j] = Ui, W[2i+1][2j+1] = Ur  (the four real products),
+//        followed, for k <= 4, by the 2^k x 2^k planes Ur + Ui and Ur - Ui,
 //      - U itself in FP64 for the VALU kernel.
 // 2. Each block is then applied in one HBM pass: complex64 states with
 //    n >= 12 by the matrix-core kernel of blockpass.hip (k_fused_mfma),
@@ -87,6 +88,15 @@ __global__ __launch_bounds__(64) void k_compose(const qcs_hip_gate *gates,
     const double v = (p == q) ? ur : (p == 0 ? -ui : ui);
     W[idx] = float(v);
   }
+  // blocks of up to 4 qubits also get Ur + Ui and Ur - Ui, each rounded
+  // once, after the real form (the three-product form of tile programs
+  // reads them)
+  if (RR * RR + 2 * R * R <= kWFloats)
+    for (int idx = threadIdx.x; idx < R * R; idx += 64) {
+      const double ur = cr[idx % R][idx / R], ui = ci[idx % R][idx / R];
+      W[RR * RR + idx] = float(ur + ui);
+      W[RR * RR + R * R + idx] = float(ur - ui);
+    }
   if (!ubuf) return;
   double *U = ubuf + size_t(blockIdx.x) * kUDoubles;
   for (int idx = threadIdx.x; idx < R * R; idx += 64) {

@@ -79,6 +79,7 @@ typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float f2v __attribute__((ext_vector_type(2)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x2 __attribute__((ext_vector_type(2)));
 
 constexpr int kPassMaxBits = 14;
 constexpr int kTileKMax = QCS_HIP_TILE_KMAX;  // qubits per block in a pass
@@ -418,7 +419,8 @@ __device__ __forceinline__ void tp_block16(const TileBlock *__restrict__ bl,
 struct K4State {
   float bv0[8];  // first-half B operands of the current block, read ahead
   f4v acc[2];    // second-half results of the previous block, not yet stored
-  int po, pr1, pr3;  // their base address, row offsets of target bits 1, 3
+  int po, pr1, pr3;  // their base address, the row offsets of the register
+                     // index bits (target bits 1, 3; 2, 3 in the three-product form)
   int pend;
 };
 
@@ -514,6 +516,113 @@ __device__ __forceinline__ void tp_block4p(const BlockScalars &d, const BlockReg
   read_k4_b0(nd, nr, st);
 }
 
+// The three-product form of a pipelined k = 4 block.  With U = Ur + i Ui
+// and a column x = xr + i xi, and the products
+//   P1 = (Ur + Ui) xr,   P2 = (Ur - Ui) xi,   P3 = -Ui (xr + xi),
+// U x = (P1 + P3) + i (P2 - P3): three real 16x16 products (four 16x16x4
+// instructions each) where the other forms do four.  The matrix cores also
+// do the additions: P3's chain runs first, and the chains of P1 and -P2
+// start from its result, so a block yields Re(U x) and -Im(U x) with no
+// vector ALU work beyond xr + xi.  Inside a pass the tile's imaginary plane
+// therefore holds -Im after every block: a block reads v = s Im(x), with
+// s = +1 for the pass's first block (the tile as loaded) and -1 after, and
+// its operands are (Ur + Ui), -s (Ur - Ui) and -Ui, with xr + xi = xr + s v.
+// The tile is written back with its imaginary plane negated.
+//
+// Lane (q, c) supplies B[q][c] = row q | t << 2 of column c for instruction
+// t, and receives rows 4q + v of each 16x16 result, which the host stages to
+// mean output row q | v << 2: a lane reads and writes the rows that lane
+// bits 4 and 5 (q) select on target bits 0 and 1.  The operands are a[p][t]
+// for product p in the order above; in place of a fourth, the lane's
+// address in each column half (lane table entry ^ the row offsets q
+// selects), staged per block, so a block needs one XOR with the wave part
+// per half.  The pipeline, the read-ahead and the `early` rule are those of
+// tp_block4p.
+__device__ __forceinline__ void load_regs4g(BlockRegs &r, int blk) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int v = 0; v < 3; ++v) r.a[v] = lds_load<f4v>(blk + (v * 64 + lane) * 16);
+  const i32x2 l = lds_load<i32x2>(blk + (3 * 64 + lane) * 16);
+  r.lp = l[0];
+  r.lp2 = l[1];
+}
+
+// B operands of one column half (l: the lane's address in it): real parts
+// of rows q | t << 2 in b[t], imaginary parts in b[4 + t]
+__device__ __forceinline__ void read_k4g(const BlockScalars &d, int l, float *b) {
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int o = l ^ d.ro(t << 2);
+    b[t] = lds_load<float>(o);
+    b[4 + t] = lds_load<float>(o + kTilePlane);
+  }
+}
+
+__device__ __forceinline__ void read_k4g_b0(const BlockScalars &d, const BlockRegs &r,
+                                            K4State &st) {
+  read_k4g(d, r.lp ^ d.up, st.bv0);
+}
+
+// the three products of one column half (b: real parts of the B rows, then
+// s times the imaginary parts): re = Re(U x), nim = -Im(U x)
+__device__ __forceinline__ void k4g_chain(const BlockRegs &r, const float *b, float sgn,
+                                          f4v &re, f4v &nim) {
+  f4v p3 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+    p3 = __builtin_amdgcn_mfma_f32_16x16x4f32(r.a[2][t], __builtin_fmaf(sgn, b[4 + t], b[t]),
+                                              p3, 0, 0, 0);
+  re = p3;
+  nim = p3;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    re = __builtin_amdgcn_mfma_f32_16x16x4f32(r.a[0][t], b[t], re, 0, 0, 0);
+    nim = __builtin_amdgcn_mfma_f32_16x16x4f32(r.a[1][t], b[4 + t], nim, 0, 0, 0);
+  }
+}
+
+// rows q | v << 2 of one half to LDS; r2, r3: row offsets of target bits 2, 3
+__device__ __forceinline__ void store_k4g(int o, int r2, int r3, const f4v &re,
+                                          const f4v &im) {
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int addr = o ^ ((v & 1) ? r2 : 0) ^ ((v & 2) ? r3 : 0);
+    lds_store<float>(addr, re[v]);
+    lds_store<float>(addr + kTilePlane, im[v]);
+  }
+}
+
+template <int NTH, int TILE>
+__device__ __forceinline__ void tp_block4g(const BlockScalars &d, const BlockRegs &r,
+                                           float sgn, const TileBlock *__restrict__ nbl,
+                                           int nblk, int wave, BlockScalars &nd, BlockRegs &nr,
+                                           K4State &st) {
+  static_assert((TILE >> 4) / 32 / (NTH / 64) == 1, "one 32-column group per wave");
+  const int l0 = r.lp ^ d.up, l1 = r.lp2 ^ d.up;  // the lane's address in each half
+  __builtin_amdgcn_sched_barrier(0);
+  f4v re, nim;
+  k4g_chain(r, st.bv0, sgn, re, nim);
+  __builtin_amdgcn_sched_barrier(0);
+  if (st.pend) store_k4g(st.po, st.pr1, st.pr3, st.acc[0], st.acc[1]);
+  float bv1[8];
+  read_k4g(d, l1, bv1);
+  nd = load_scalars(nbl, wave);
+  __builtin_amdgcn_sched_barrier(0);
+  k4g_chain(r, bv1, sgn, st.acc[0], st.acc[1]);
+  __builtin_amdgcn_sched_barrier(0);
+  load_regs4g(nr, nblk);
+  store_k4g(l0, d.rb[2], d.rb[3], re, nim);
+  st.po = l1;
+  st.pr1 = d.rb[2];
+  st.pr3 = d.rb[3];
+  st.pend = d.early;
+  if (!d.early) {
+    store_k4g(st.po, st.pr1, st.pr3, st.acc[0], st.acc[1]);
+    if (d.sync) __syncthreads();
+  }
+  read_k4g_b0(nd, nr, st);
+}
+
 // One FP32 block of the pass (index b), with the next block's values read
 // into (nd, nr); the block after the last is the pass's first (next tile).
 template <int NTH, int TILE>
@@ -607,7 +716,10 @@ __device__ __forceinline__ void tp_block_split(const TileBlock *__restrict__ bl,
 // The descriptor pointers are __restrict__ so the compiler may read them
 // through the scalar cache (they are uniform and never written by the
 // kernel); vector loads of them would wait behind the tile prefetch.
-template <int TB, int NTH, bool SPLIT, bool K4P>
+// K4: how FP32 k = 4 blocks run -- 0: the 32x32 chain (k <= 3 blocks on
+// the 16-column form), 1: pipelined halves, 2: pipelined three-product
+// halves (1 and 2 run every block as a k = 4 block)
+template <int TB, int NTH, bool SPLIT, int K4>
 __global__ __launch_bounds__(NTH) void k_tile_program(
     float2 *__restrict__ a, const TilePass *__restrict__ passes, int pass_index,
     const TileBlock *__restrict__ blocks, const float *__restrict__ wbuf,
@@ -636,7 +748,26 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
     const int R = 1 << bl.k, RR = 2 * R, row = ln & 31, hh = ln >> 5;
     const float *W = wbuf + size_t(bl.wslot) * kTileWFloats;
     float w = (row < RR && s < R) ? W[row * RR + 2 * s + hh] : 0.0f;
-    if (!SPLIT && (bl.k <= 3 || K4P)) {
+    if (!SPLIT && K4 == 2) {
+      // three-product form: a[p][t] = A[i][qq] of product p (Ur + Ui,
+      // -s (Ur - Ui) with s = +1 for the pass's first block and -1 after,
+      // -Ui), instruction t; D row i = 4 qi + vi means output row qi | vi << 2
+      // and K-index qq input row qq | t << 2.  a[3][0] and a[3][1]: the lane's
+      // address in column half 0 and 1, lane table entry (ln & 15) | 16 h ^
+      // the row offsets of target bits 0 and 1 on lane bits 4 and 5.
+      const int i = ln & 15, qq = ln >> 4, p = s >> 2, t = s & 3;
+      const int ro = (i >> 2) | ((i & 3) << 2), si = qq | (t << 2);
+      if (p < 3) {
+        const bool in = ro < R && si < R;
+        w = !in ? 0.0f
+            : p == 0 ? W[RR * RR + ro * R + si]
+            : p == 1 ? (b == 0 ? -1.0f : 1.0f) * W[RR * RR + R * R + ro * R + si]
+                     : -W[(2 * ro + 1) * RR + 2 * si];
+      } else {
+        const int lrows = ((ln & 16) ? bl.rowoff[1] : 0) ^ ((ln & 32) ? bl.rowoff[2] : 0);
+        w = t < 2 ? __int_as_float(bl.lanecol[(ln & 15) | (t << 4)] ^ lrows) : 0.0f;
+      }
+    } else if (!SPLIT && (bl.k <= 3 || K4 == 1)) {
       // the 16-column form (tp_block16, tp_block4p): instruction sp = s & 7
       // of row block rb = s >> 3, A[i][qq] with i = 4 qi + vi holding output
       // row (qi & 1) | (vi >> 1) << 1 | (qi >> 1) << 2 | rb << 3, part vi & 1,
@@ -724,7 +855,9 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
   BlockRegs ra, rb;
   if (!SPLIT) {
     sa = load_scalars(&blocks[ps.first_block], wave);
-    if (K4P)
+    if (K4 == 2)
+      load_regs4g(ra, OPS);
+    else if (K4 == 1)
       load_regs4p(ra, OPS);
     else
       load_regs(ra, OPS);
@@ -773,16 +906,24 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
       // two register sets taken in turn, so the values read one block ahead
       // need no copies; (sa, ra) hold the pass's first block on entry
       const TileBlock *fb = &blocks[ps.first_block];
-      if constexpr (K4P) {
+      if constexpr (K4 != 0) {
         // every block is a k = 4 block (the host pads smaller ones); one set
         // of per-lane values, read in place once a block's last chain is
         // issued; the scalars alternate between two sets
         K4State st;
         st.pend = 0;
-        read_k4_b0(sa, ra, st);
+        if constexpr (K4 == 2)
+          read_k4g_b0(sa, ra, st);
+        else
+          read_k4_b0(sa, ra, st);
         for (int b = 0; b < nb; ++b) {
           const int bn = b + 1 < nb ? b + 1 : 0;
-          tp_block4p<NTH, TILE>(sa, ra, fb + bn, OPS + bn * kTileStrideFp32, wave, sb, ra, st);
+          if constexpr (K4 == 2)
+            tp_block4g<NTH, TILE>(sa, ra, b == 0 ? 1.0f : -1.0f, fb + bn,
+                                  OPS + bn * kTileStrideFp32, wave, sb, ra, st);
+          else
+            tp_block4p<NTH, TILE>(sa, ra, fb + bn, OPS + bn * kTileStrideFp32, wave, sb, ra,
+                                  st);
           sa = sb;
         }
       } else {
@@ -803,7 +944,10 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
       const int s0 = slot_t ^ swz(it << TBITS);
-      o[it] = make_float4(re[s0], im[s0], re[s0 ^ 1], im[s0 ^ 1]);
+      if constexpr (K4 == 2)  // the three-product form leaves -Im in the tile
+        o[it] = make_float4(re[s0], -im[s0], re[s0 ^ 1], -im[s0 ^ 1]);
+      else
+        o[it] = make_float4(re[s0], im[s0], re[s0 ^ 1], im[s0 ^ 1]);
     }
     __syncthreads();  // the finished tile is out of LDS
     if (next < tend) {
@@ -836,7 +980,7 @@ static int grow_dev(void **p, size_t *cap, size_t need) {
   return 0;
 }
 
-template <int NTH, bool SPLIT, bool K4P>
+template <int NTH, bool SPLIT, int K4>
 static int launch_tp(qcs_hip_state *s, const TilePass *dpass, int pi,
                      const TileBlock *dblocks, uint64_t ntiles, int nblocks) {
   // once per kernel instance (states may launch from separate threads)
@@ -844,7 +988,7 @@ static int launch_tp(qcs_hip_state *s, const TilePass *dpass, int pi,
   static int attr_rc = 0;
   std::call_once(once, [] {
     const void *fn =
-        reinterpret_cast<const void *>(&k_tile_program<kTileBitsFixed, NTH, SPLIT, K4P>);
+        reinterpret_cast<const void *>(&k_tile_program<kTileBitsFixed, NTH, SPLIT, K4>);
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        int(kLdsPerCu));
     if (e != hipSuccess) {
@@ -868,7 +1012,7 @@ static int launch_tp(qcs_hip_state *s, const TilePass *dpass, int pi,
   const uint64_t per_cu = kLdsPerCu / lds;  // work-groups that fit per CU
   uint64_t grid = uint64_t(s->tile_grid) * (per_cu > 1 ? 2 : 1);
   if (grid > ntiles) grid = ntiles;
-  hipLaunchKernelGGL((k_tile_program<kTileBitsFixed, NTH, SPLIT, K4P>), dim3(unsigned(grid)),
+  hipLaunchKernelGGL((k_tile_program<kTileBitsFixed, NTH, SPLIT, K4>), dim3(unsigned(grid)),
                      dim3(NTH), lds, s->stream, static_cast<float2 *>(s->vec),
                      dpass, pi, dblocks, s->wbuf, ntiles);
   return launch_check("k_tile_program");
@@ -991,12 +1135,11 @@ static void make_tile_block(const qcs_hip_pass &ps, const qcs_hip_block &blk, in
 }
 
 // Descriptors of one pass.  Blocks are taken in runs that share their wave
-// bits, and only the last block of a run is followed by a barrier: a run
-// grows while the union of its targets leaves nwb tile bits free.  The wave
-// bits of a run are the choice with the fewest bank conflicts over its
-// blocks (a shared choice can leave some of them 2-way conflicted: on this
-// kernel a barrier costs more than that).  With FP32 math, blocks of k <= 3
-// run in the 16-column form, whose conflicts also depend on the first target.
+// bits, and only the last block of a run is followed by a barrier.  The
+// wave bits of each block (and the half bit of the pipelined k = 4 forms)
+// trade barriers against LDS bank conflicts, by dynamic programming over the
+// pass.  With FP32 math, blocks of k <= 3 run in the 16-column form, whose
+// conflicts also depend on the first target.
 static void layout_pass(const qcs_hip_pass &ps, const qcs_hip_block *blocks, int nwb,
                         bool split, bool k4p, TileBlock *tb) {
   // conflict factor of every set of free positions (and, for the 16-column
@@ -1009,68 +1152,146 @@ static void layout_pass(const qcs_hip_pass &ps, const qcs_hip_block *blocks, int
   std::call_once(once, [] {
     int lanes[5];
     for (int a = 0; a < kSets; ++a) {
-      const bool enough = __builtin_popcount(unsigned(a)) >= 5;
-      factor[a] = enough ? pick_lanes(a, lanes) : 0;
-      for (int c = 0; c < 5; ++c) factor16[c][a] = enough ? pick_lanes16(a, c, lanes) : 0;
+      // 5 lanes; the 16-column form needs 4 (and a half bit, which the
+      // pipelined forms take out of the set first)
+      const int np = __builtin_popcount(unsigned(a));
+      factor[a] = np >= 5 ? pick_lanes(a, lanes) : 0;
+      for (int c = 0; c < 5; ++c) factor16[c][a] = np >= 4 ? pick_lanes16(a, c, lanes) : 0;
     }
     for (int m = 0; m < kSets; ++m)
       if (__builtin_popcount(unsigned(m)) <= 4) masks[__builtin_popcount(unsigned(m))].push_back(m);
   });
-  const int end = ps.first_block + ps.num_blocks;
   auto form16 = [&](int i) { return !split && (blocks[i].k <= 3 || k4p); };
-  std::vector<int> tm(size_t(ps.num_blocks));
-  for (int i = 0; i < ps.num_blocks; ++i) tm[i] = target_mask(ps, blocks[ps.first_block + i]);
-  for (int b = ps.first_block; b < end;) {
-    int u = tm[b - ps.first_block], e = b + 1;
-    for (; e < end; ++e) {
-      const int v = u | tm[e - ps.first_block];
-      if (__builtin_popcount(unsigned(v)) > kTileBitsFixed - nwb) break;
-      u = v;
-    }
-    int best = -1, best_cost = 0;
-    for (int m : masks[nwb]) {
-      if (m & u) continue;
-      int cost = 0;
-      for (int i = b; i < e; ++i) {
-        const int t = tm[i - ps.first_block], a = (kSets - 1) & ~t & ~m;
-        cost += form16(i) ? factor16[bank_bit(__builtin_ctz(unsigned(t)))][a] : factor[a];
+  const int nblk = ps.num_blocks;
+  std::vector<int> tm(static_cast<size_t>(nblk));
+  for (int i = 0; i < nblk; ++i) tm[i] = target_mask(ps, blocks[ps.first_block + i]);
+  // Wave bits and half bit of every block by dynamic programming over the
+  // pass's blocks.  State (j, h): the block runs on wave bits cand[j] and,
+  // in the pipelined k = 4 forms, half bit h (lane bit 4; one state h = 0
+  // otherwise).  A block continues the previous block's run when both have
+  // the same wave bits (no barrier between them), and reads ahead when it
+  // also has the same half bit.  Costs: kRunCost per barrier, kNoAheadCost
+  // per block of a run that cannot read ahead, kConflictCost per extra way
+  // of a block's bank conflicts (a 4-way conflicted block makes LDS the
+  // bottleneck of the CU; a 2-way one mostly hides under the matrix cores).
+  const std::vector<int> &cand = masks[nwb];
+  const int nm = int(cand.size()), nh = k4p ? kTileBitsFixed : 1;
+  // (weights measured on the 30-qubit layered circuit: these beat sharing
+  // wave and half bits over long runs by ~3%, and one barrier per block)
+  constexpr int kRunCost = 6, kNoAheadCost = 1, kConflictCost = 4;
+  constexpr int kInf = 1 << 29;
+  // conflict cost of block i on wave bits mk with half bit h (-1 if h is
+  // not free)
+  auto conflict = [&](int i, int mk, int h) {
+    const int t = tm[i];
+    int a = (kSets - 1) & ~t & ~mk;
+    int f;
+    if (form16(ps.first_block + i)) {
+      if (k4p) {
+        if (!((a >> h) & 1)) return -1;
+        a &= ~(1 << h);
       }
-      if (best < 0 || cost < best_cost) {
-        best = m;
-        best_cost = cost;
-      }
+      f = factor16[bank_bit(__builtin_ctz(unsigned(t)))][a];
+    } else {
+      f = factor[a];
     }
-    // pipelined k = 4 blocks: one half bit (lane bit 4) for the whole run
-    // where a position is free in all its blocks, the one with the fewest
-    // bank conflicts, so consecutive blocks can read ahead
-    int half = -1;
-    if (k4p) {
-      int best_h = 0;
-      const int freepos = (kSets - 1) & ~u & ~best;
-      for (int h = 0; h < kTileBitsFixed; ++h) {
-        if (!((freepos >> h) & 1)) continue;
-        int cost = 0, lanes[5];
-        for (int i = b; i < e; ++i) {
-          const int t = tm[i - ps.first_block], a = (kSets - 1) & ~t & ~best;
-          cost += form16(i) ? pick_lanes16(a, bank_bit(__builtin_ctz(unsigned(t))), lanes, h)
-                            : factor[a];
+    return kConflictCost * (f - 1);
+  };
+  const size_t ns = size_t(nm) * size_t(nh);
+  // from[i][s]: -1 when block i continues state s of block i - 1, else the
+  // state of block i - 1 it follows (a new half bit or a new run)
+  std::vector<int> cost(ns * size_t(nblk), kInf), from(ns * size_t(nblk), -1);
+  std::vector<int> best_of_mask(static_cast<size_t>(nm));
+  std::vector<int> arg_of_mask(static_cast<size_t>(nm));
+  int prev_best = 0, prev_arg = -1;
+  for (int i = 0; i < nblk; ++i) {
+    int best = kInf, arg = -1;
+    int *c = cost.data() + ns * size_t(i);
+    int *fr = from.data() + ns * size_t(i);
+    const int *pc = i > 0 ? cost.data() + ns * size_t(i - 1) : nullptr;
+    if (pc)
+      for (int j = 0; j < nm; ++j) {
+        best_of_mask[j] = kInf;
+        for (int h = 0; h < nh; ++h)
+          if (pc[j * nh + h] < best_of_mask[j]) {
+            best_of_mask[j] = pc[j * nh + h];
+            arg_of_mask[j] = j * nh + h;
+          }
+      }
+    for (int j = 0; j < nm; ++j) {
+      if (cand[j] & tm[i]) continue;
+      for (int h = 0; h < nh; ++h) {
+        const int cf = conflict(i, cand[j], h);
+        if (cf < 0) continue;
+        const int sidx = j * nh + h;
+        int v = prev_best + kRunCost, f = prev_arg;  // a new run
+        if (pc && best_of_mask[j] + kNoAheadCost < v) {
+          v = best_of_mask[j] + kNoAheadCost;  // same run, no read-ahead
+          f = arg_of_mask[j];
         }
-        if (half < 0 || cost < best_h) {
-          half = h;
-          best_h = cost;
+        if (pc && pc[sidx] <= v) {
+          v = pc[sidx];  // same run, reads ahead
+          f = -1;
+        }
+        c[sidx] = v + cf;
+        fr[sidx] = f;
+        if (c[sidx] < best) {
+          best = c[sidx];
+          arg = sidx;
         }
       }
     }
-    for (int i = b; i < e; ++i)
-      make_tile_block(ps, blocks[i], i, best, nwb, i == e - 1, form16(i), half, &tb[i]);
-    // a k = 4 block reads the next block's first half ahead when that block is
-    // also k = 4, in this run, with the same half bit
-    for (int i = b; i + 1 < e; ++i)
-      tb[i].early = (k4p && blocks[i].k == 4 && blocks[i + 1].k == 4 &&
-                     tb[i].lanecol[16] == tb[i + 1].lanecol[16])
-                        ? 1
-                        : 0;
-    b = e;
+    prev_best = best;
+    prev_arg = arg;
+  }
+  std::vector<int> wmask(static_cast<size_t>(nblk)), whalf(static_cast<size_t>(nblk));
+  for (int i = nblk - 1, sidx = prev_arg; i >= 0; --i) {
+    wmask[i] = cand[sidx / nh];
+    whalf[i] = k4p ? sidx % nh : -1;
+    const int f = from[ns * size_t(i) + size_t(sidx)];
+    if (f >= 0) sidx = f;
+  }
+  for (int i = 0; i < nblk; ++i) {
+    const int bi = ps.first_block + i;
+    const bool last = i + 1 == nblk || wmask[i + 1] != wmask[i];
+    make_tile_block(ps, blocks[bi], bi, wmask[i], nwb, last, form16(bi), whalf[i], &tb[bi]);
+  }
+  // a k = 4 block reads the next block's first half ahead when that block is
+  // also k = 4, in this run, with the same half bit
+  for (int i = 0; i + 1 < nblk; ++i) {
+    const int bi = ps.first_block + i;
+    tb[bi].early = (k4p && blocks[bi].k == 4 && blocks[bi + 1].k == 4 && !tb[bi].sync &&
+                    tb[bi].lanecol[16] == tb[bi + 1].lanecol[16])
+                       ? 1
+                       : 0;
+  }
+}
+
+// Blocks of fewer than 4 qubits widened to 4 (their unitary composes to
+// U x I on the added targets), for the kernels that run every block as a
+// k = 4 block.  The added targets are taken from the neighbouring blocks'
+// targets where possible, then from the pass bits, highest first, so no run
+// of blocks needs more tile bits than before.
+static void pad_blocks(const qcs_hip_pass *passes, int np, const qcs_hip_block *blocks,
+                       int nb, std::vector<qcs_hip_block> *out) {
+  out->assign(blocks, blocks + nb);
+  for (int p = 0; p < np; ++p) {
+    const int b0 = passes[p].first_block, b1 = b0 + passes[p].num_blocks;
+    for (int b = b0; b < b1; ++b) {
+      qcs_hip_block &bl = (*out)[size_t(b)];
+      if (bl.k >= QCS_HIP_TILE_KMAX) continue;
+      int cand[3 * QCS_HIP_TILE_KMAX + QCS_HIP_TILE_BITS], nc = 0;
+      for (int nbh : {b - 1, b + 1})
+        if (nbh >= b0 && nbh < b1)
+          for (int q = 0; q < blocks[nbh].k; ++q) cand[nc++] = blocks[nbh].targets[q];
+      for (int j = passes[p].tile_bits - 1; j >= 0; --j) cand[nc++] = passes[p].bits[j];
+      for (int c = 0; c < nc && bl.k < QCS_HIP_TILE_KMAX; ++c) {
+        bool have = false;
+        for (int q = 0; q < bl.k; ++q) have = have || bl.targets[q] == cand[c];
+        if (!have) bl.targets[bl.k++] = cand[c];
+      }
+      std::sort(bl.targets, bl.targets + bl.k);
+    }
   }
 }
 
@@ -1090,29 +1311,11 @@ int tiles_apply(qcs_hip_state *s, const qcs_hip_pass *passes, int np,
   // more targets (their unitary composes to U x I on them), taken from the
   // neighbouring blocks' targets where possible so no run needs more bits
   const bool k4p = !split_math && s->tile_threads == 1024 &&
-                   s->tile_k4 == QCS_HIP_TILE_K4_HALVES;
+                   (s->tile_k4 == QCS_HIP_TILE_K4_HALVES || s->tile_k4 == QCS_HIP_TILE_K4_GAUSS);
   std::vector<qcs_hip_block> padded;
   const qcs_hip_block *lay = blocks;
   if (k4p) {
-    padded.assign(blocks, blocks + nb);
-    for (int p = 0; p < np; ++p) {
-      const int b0 = passes[p].first_block, b1 = b0 + passes[p].num_blocks;
-      for (int b = b0; b < b1; ++b) {
-        qcs_hip_block &bl = padded[size_t(b)];
-        if (bl.k >= QCS_HIP_TILE_KMAX) continue;
-        int cand[3 * QCS_HIP_TILE_KMAX + QCS_HIP_TILE_BITS], nc = 0;
-        for (int nbh : {b - 1, b + 1})
-          if (nbh >= b0 && nbh < b1)
-            for (int q = 0; q < blocks[nbh].k; ++q) cand[nc++] = blocks[nbh].targets[q];
-        for (int j = passes[p].tile_bits - 1; j >= 0; --j) cand[nc++] = passes[p].bits[j];
-        for (int c = 0; c < nc && bl.k < QCS_HIP_TILE_KMAX; ++c) {
-          bool have = false;
-          for (int q = 0; q < bl.k; ++q) have = have || bl.targets[q] == cand[c];
-          if (!have) bl.targets[bl.k++] = cand[c];
-        }
-        std::sort(bl.targets, bl.targets + bl.k);
-      }
-    }
+    pad_blocks(passes, np, blocks, nb, &padded);
     lay = padded.data();
   }
   char *stage = nullptr;
@@ -1149,12 +1352,14 @@ int tiles_apply(qcs_hip_state *s, const qcs_hip_pass *passes, int np,
     timing_begin(s, QCS_HIP_K_FUSED, 2.0 * N * double(s->amp_bytes), flops, &tl);
     const int nbp = passes[p].num_blocks;
     if (s->tile_threads == 1024)
-      rc = split ? launch_tp<1024, true, false>(s, dp, p, db, ntiles, nbp)
-           : k4p ? launch_tp<1024, false, true>(s, dp, p, db, ntiles, nbp)
-                 : launch_tp<1024, false, false>(s, dp, p, db, ntiles, nbp);
+      rc = split ? launch_tp<1024, true, 0>(s, dp, p, db, ntiles, nbp)
+           : !k4p ? launch_tp<1024, false, 0>(s, dp, p, db, ntiles, nbp)
+           : s->tile_k4 == QCS_HIP_TILE_K4_GAUSS
+               ? launch_tp<1024, false, 2>(s, dp, p, db, ntiles, nbp)
+               : launch_tp<1024, false, 1>(s, dp, p, db, ntiles, nbp);
     else
-      rc = split ? launch_tp<512, true, false>(s, dp, p, db, ntiles, nbp)
-                 : launch_tp<512, false, false>(s, dp, p, db, ntiles, nbp);
+      rc = split ? launch_tp<512, true, 0>(s, dp, p, db, ntiles, nbp)
+                 : launch_tp<512, false, 0>(s, dp, p, db, ntiles, nbp);
     if (rc) return rc;
     if ((rc = timing_end(s, &tl))) return rc;
   }

@@ -238,9 +238,9 @@ static int autotune_tiles(t_q_circuit *c, const char *dev) {
   for (th = 512; th <= 1024; th *= 2)
     for (g = 1; g <= 2; g++)
       for (k = 3; k <= QCS_HIP_TILE_KMAX; k++)
-        /* the pipelined k = 4 form exists for FP32 math on 1024 threads */
+        /* the pipelined k = 4 forms exist for FP32 math on 1024 threads */
         for (form = QCS_HIP_TILE_K4_CHAIN;
-             form <= ((th == 1024 && !split) ? QCS_HIP_TILE_K4_HALVES : QCS_HIP_TILE_K4_CHAIN);
+             form <= ((th == 1024 && !split) ? QCS_HIP_TILE_K4_GAUSS : QCS_HIP_TILE_K4_CHAIN);
              form++) {
           if ((rc = set_tile_shape(c->state, th, g * cus, form))) return rc;
           if ((rc = time_tile_plan(c, k, max_blocks, &ms))) return rc;

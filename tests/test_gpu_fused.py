@@ -18,6 +18,8 @@ import helpers as HP
 import qcs_amd
 from qcs_amd import circuits as C
 
+K4_FORMS = (qcs_amd.TILE_K4_CHAIN, qcs_amd.TILE_K4_HALVES, qcs_amd.TILE_K4_GAUSS)
+
 pytestmark = pytest.mark.gpu
 
 
@@ -137,11 +139,11 @@ def test_fused_random_circuits_vs_oracle_and_numpy(qlib, n):
 @pytest.mark.parametrize("n,threads", [(16, 512), (17, 1024), (20, 512), (20, 1024)])
 def test_tile_programs_equal_single_block_passes(qlib, n, threads):
     """Tile programs apply the same composed blocks, in the same order, as
-    one pass per block.  Blocks of four qubits use the same MFMA and
-    accumulation order in both kernels; smaller ones run on the 16x16x4
-    instruction in tile programs (no padding to 32 rows), which rounds
-    differently, so the states agree to FP32 rounding: a block applied out of
-    order, to the wrong bits or with a wrong operand would differ by O(1)."""
+    one pass per block.  They run them in other instruction forms (16x16x4
+    halves, three real products per complex product at 1024 threads) than
+    the one-block pass, which rounds differently, so the states agree to
+    FP32 rounding: a block applied out of order, to the wrong bits or with a
+    wrong operand would differ by O(1)."""
     from test_fusion_plan import pass_plan, to_gates  # noqa: F401
     hip, _ = qlib
     print(f"[fused] tile programs n={n} threads={threads}", flush=True)
@@ -170,13 +172,15 @@ def test_tile_programs_equal_single_block_passes(qlib, n, threads):
 
 @pytest.mark.parametrize("n,seed", [(16, 1), (19, 2), (22, 3)])
 def test_tile_k4_halves_form(qlib, n, seed):
-    """The pipelined form of k = 4 blocks (two 16-column halves on the 16x16x4
-    instruction; smaller blocks padded to four targets; a block's second half
-    stored behind the next block's first half, whose B operands are read
-    ahead) applies the same blocks as the 32x32 chain: the states agree to
-    FP32 rounding and stay within the form's 1 - F of the FP64 simulation.
-    A block read ahead of what it depends on, a half stored to the wrong
-    columns or a block padded on a wrong target would differ by O(1)."""
+    """The pipelined forms of k = 4 blocks (two 16-column halves on the
+    16x16x4 instruction; smaller blocks padded to four targets; a block's
+    second half stored behind the next block's first half, whose B operands
+    are read ahead), with four real products per complex product or three
+    (Gauss), apply the same blocks as the 32x32 chain: the states agree to
+    FP32 rounding and stay within 1 - F <= 1e-6 of the FP64 simulation.  A
+    block read ahead of what it depends on, a half stored to the wrong
+    columns, a block padded on a wrong target or a wrong sign or operand in
+    the three-product combination would differ by O(1)."""
     from test_fusion_plan import pass_plan
     hip, _ = qlib
     print(f"[fused] k4 halves n={n}", flush=True)
@@ -191,31 +195,34 @@ def test_tile_k4_halves_form(qlib, n, seed):
         B = (qcs_amd.HipBlock * len(blocks))(*blocks)
         G = (qcs_amd.HipGate * len(ordered))(*ordered)
         psi = rand_state(n, np.random.default_rng(n + seed))
-        with qcs_amd.Circuit(n, qcs_amd.PLAIN_FP32) as a, \
-                qcs_amd.Circuit(n, qcs_amd.PLAIN_FP32) as b:
-            for st, form in ((a, qcs_amd.TILE_K4_CHAIN), (b, qcs_amd.TILE_K4_HALVES)):
+        ref = HP.simulate(n, o, psi.astype(np.complex64).astype(complex))
+        got = {}
+        for form in K4_FORMS:
+            with qcs_amd.Circuit(n, qcs_amd.PLAIN_FP32) as st:
                 st.set_amplitudes(psi)
                 assert hip.qcs_hip_set_tuning(st.device_state, qcs_amd.TUNE_TILE_K4_FORM,
                                               form) == 0
                 rc = hip.qcs_hip_apply_passes(st.device_state, P, len(passes), B,
                                               len(blocks), G, len(ordered))
                 assert rc == 0, hip.qcs_hip_last_error()
-            chain, halves = a.amplitudes(), b.amplitudes()
-        assert np.linalg.norm(halves - chain) <= 1e-5
-        ref = HP.simulate(n, o, psi.astype(np.complex64).astype(complex))
-        assert 1 - abs(np.vdot(ref, halves)) ** 2 / np.vdot(halves, halves).real <= 1e-6
+                got[form] = st.amplitudes()
+        for form in (qcs_amd.TILE_K4_HALVES, qcs_amd.TILE_K4_GAUSS):
+            x = got[form]
+            assert np.linalg.norm(x - got[qcs_amd.TILE_K4_CHAIN]) <= 1e-5, form
+            assert 1 - abs(np.vdot(ref, x)) ** 2 / np.vdot(x, x).real <= 1e-6, form
     # through the qc_* API: a tile-program circuit with the form set
     ops = C.layered(n, 8, seed + 10)
-    with qcs_amd.Circuit(n) as ref, \
-            qcs_amd.Circuit(n, qcs_amd.FUSED_FP32 | qcs_amd.FUSE_TILES) as f:
-        assert hip.qcs_hip_set_tuning(f.device_state, qcs_amd.TUNE_TILE_K4_FORM,
-                                      qcs_amd.TILE_K4_HALVES) == 0
-        ref.apply(ops)
-        f.apply(ops)
-        omf, mx = f.compare_with(ref)
-        assert omf <= 1e-6 and mx < 1e-5, (omf, mx)
+    for form in (qcs_amd.TILE_K4_HALVES, qcs_amd.TILE_K4_GAUSS):
+        with qcs_amd.Circuit(n) as ref, \
+                qcs_amd.Circuit(n, qcs_amd.FUSED_FP32 | qcs_amd.FUSE_TILES) as f:
+            assert hip.qcs_hip_set_tuning(f.device_state, qcs_amd.TUNE_TILE_K4_FORM,
+                                          form) == 0
+            ref.apply(ops)
+            f.apply(ops)
+            omf, mx = f.compare_with(ref)
+            assert omf <= 1e-6 and mx < 1e-5, (form, omf, mx)
     with qcs_amd.Circuit(n, qcs_amd.PLAIN_FP32) as a:
-        assert hip.qcs_hip_set_tuning(a.device_state, qcs_amd.TUNE_TILE_K4_FORM, 2) == -1
+        assert hip.qcs_hip_set_tuning(a.device_state, qcs_amd.TUNE_TILE_K4_FORM, 3) == -1
 
 
 def test_apply_passes_rejects_bad_plans(qlib):
