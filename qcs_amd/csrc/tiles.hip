@@ -419,8 +419,7 @@ __device__ __forceinline__ void tp_block16(const TileBlock *__restrict__ bl,
 struct K4State {
   float bv0[8];  // first-half B operands of the current block, read ahead
   f4v acc[2];    // second-half results of the previous block, not yet stored
-  int po, pr1, pr3;  // their base address, the row offsets of the register
-                     // index bits (target bits 1, 3; 2, 3 in the three-product form)
+  int po, pr1, pr3;  // their base address, row offsets of target bits 1, 3
   int pend;
 };
 
@@ -536,16 +535,35 @@ __device__ __forceinline__ void tp_block4p(const BlockScalars &d, const BlockReg
 // for product p in the order above; in place of a fourth, the lane's
 // address in each column half (lane table entry ^ the row offsets q
 // selects), staged per block, so a block needs one XOR with the wave part
-// per half.  The pipeline, the read-ahead and the `early` rule are those of
-// tp_block4p.
-__device__ __forceinline__ void load_regs4g(BlockRegs &r, int blk) {
+// per half.
+//
+// Schedule.  Both halves' B operands of a block are read at the end of the
+// block before it, so its two chains (24 instructions) issue back to back
+// with no LDS wait between them; the next block's lane addresses are read
+// at the top of the block, its scalars during the first chain and its A
+// operands once the second chain is issued.  Under the `early` rule (the
+// next block is in the same run, with the same half bit: its first half
+// reads only what this block's first half wrote) the next block's
+// first-half reads go out before this block's second half is stored.
+__device__ __forceinline__ void load_a4g(BlockRegs &r, int blk) {
   const int lane = threadIdx.x & 63;
 #pragma unroll
   for (int v = 0; v < 3; ++v) r.a[v] = lds_load<f4v>(blk + (v * 64 + lane) * 16);
-  const i32x2 l = lds_load<i32x2>(blk + (3 * 64 + lane) * 16);
+}
+__device__ __forceinline__ i32x2 load_lanes4g(int blk) {
+  return lds_load<i32x2>(blk + (3 * 64 + (threadIdx.x & 63)) * 16);
+}
+__device__ __forceinline__ void load_regs4g(BlockRegs &r, int blk) {
+  load_a4g(r, blk);
+  const i32x2 l = load_lanes4g(blk);
   r.lp = l[0];
   r.lp2 = l[1];
 }
+
+// both halves' B operands of a block, read ahead
+struct K4GState {
+  float b0[8], b1[8];
+};
 
 // B operands of one column half (l: the lane's address in it): real parts
 // of rows q | t << 2 in b[t], imaginary parts in b[4 + t]
@@ -558,9 +576,10 @@ __device__ __forceinline__ void read_k4g(const BlockScalars &d, int l, float *b)
   }
 }
 
-__device__ __forceinline__ void read_k4g_b0(const BlockScalars &d, const BlockRegs &r,
-                                            K4State &st) {
-  read_k4g(d, r.lp ^ d.up, st.bv0);
+__device__ __forceinline__ void read_k4g_both(const BlockScalars &d, const BlockRegs &r,
+                                              K4GState &st) {
+  read_k4g(d, r.lp ^ d.up, st.b0);
+  read_k4g(d, r.lp2 ^ d.up, st.b1);
 }
 
 // the three products of one column half (b: real parts of the B rows, then
@@ -596,31 +615,33 @@ template <int NTH, int TILE>
 __device__ __forceinline__ void tp_block4g(const BlockScalars &d, const BlockRegs &r,
                                            float sgn, const TileBlock *__restrict__ nbl,
                                            int nblk, int wave, BlockScalars &nd, BlockRegs &nr,
-                                           K4State &st) {
+                                           K4GState &st) {
   static_assert((TILE >> 4) / 32 / (NTH / 64) == 1, "one 32-column group per wave");
   const int l0 = r.lp ^ d.up, l1 = r.lp2 ^ d.up;  // the lane's address in each half
+  const i32x2 nl = load_lanes4g(nblk);
   __builtin_amdgcn_sched_barrier(0);
-  f4v re, nim;
-  k4g_chain(r, st.bv0, sgn, re, nim);
+  f4v re0, nim0, re1, nim1;
+  k4g_chain(r, st.b0, sgn, re0, nim0);
   __builtin_amdgcn_sched_barrier(0);
-  if (st.pend) store_k4g(st.po, st.pr1, st.pr3, st.acc[0], st.acc[1]);
-  float bv1[8];
-  read_k4g(d, l1, bv1);
   nd = load_scalars(nbl, wave);
   __builtin_amdgcn_sched_barrier(0);
-  k4g_chain(r, bv1, sgn, st.acc[0], st.acc[1]);
+  k4g_chain(r, st.b1, sgn, re1, nim1);
   __builtin_amdgcn_sched_barrier(0);
-  load_regs4g(nr, nblk);
-  store_k4g(l0, d.rb[2], d.rb[3], re, nim);
-  st.po = l1;
-  st.pr1 = d.rb[2];
-  st.pr3 = d.rb[3];
-  st.pend = d.early;
-  if (!d.early) {
-    store_k4g(st.po, st.pr1, st.pr3, st.acc[0], st.acc[1]);
+  load_a4g(nr, nblk);
+  nr.lp = nl[0];
+  nr.lp2 = nl[1];
+  store_k4g(l0, d.rb[2], d.rb[3], re0, nim0);
+  if (d.early) {
+    read_k4g(nd, nr.lp ^ nd.up, st.b0);
+    store_k4g(l1, d.rb[2], d.rb[3], re1, nim1);
+    read_k4g(nd, nr.lp2 ^ nd.up, st.b1);
+  } else {
+    // (after a pass's last block the reads serve nothing: the next tile's
+    // first block reads again once the tile is in LDS)
+    store_k4g(l1, d.rb[2], d.rb[3], re1, nim1);
     if (d.sync) __syncthreads();
+    read_k4g_both(nd, nr, st);
   }
-  read_k4g_b0(nd, nr, st);
 }
 
 // One FP32 block of the pass (index b), with the next block's values read
@@ -844,11 +865,17 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
   // count against the same in-order vector-memory counter as loads, so a
   // wait for the prefetch placed after the stores would also wait for the
   // stores to reach memory.
-  float4 v[IT];
+  // DEPTH tiles are in flight: the three-product form loads tile t + 2
+  // while it computes tile t (two register sets taken in turn), so a pass
+  // whose tiles are scattered over many pages still streams at the HBM rate.
+  constexpr int DEPTH = (!SPLIT && K4 == 2) ? 2 : 1;
+  float4 v0[IT], v1[IT];
   uint64_t base = deposit(tile);
+  auto step_base = [&](uint64_t b) { return ((b | ~cmask) + dinc) & cmask; };
+  uint64_t nbase = step_base(base), nbase2 = step_base(nbase);
   if (tile < tend) {
 #pragma unroll
-    for (int it = 0; it < IT; ++it) v[it] = ld_nt(a + (base | it_addr(it) | addr_t));
+    for (int it = 0; it < IT; ++it) v0[it] = ld_nt(a + (base | it_addr(it) | addr_t));
   }
   __syncthreads();  // operands staged
   BlockScalars sa, sb;
@@ -866,21 +893,31 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
       const int s0 = slot_t ^ swz(it << TBITS);  // slot of e; e+1 is s0 ^ 1
-      re[s0] = v[it].x;
-      im[s0] = v[it].y;
-      re[s0 ^ 1] = v[it].z;
-      im[s0 ^ 1] = v[it].w;
+      re[s0] = v0[it].x;
+      im[s0] = v0[it].y;
+      re[s0 ^ 1] = v0[it].z;
+      im[s0 ^ 1] = v0[it].w;
     }
   }
   __syncthreads();
-
-  for (; tile < tend; tile += G8) {
-    const uint64_t next = tile + G8;
-    const uint64_t nbase = ((base | ~cmask) + dinc) & cmask;
-    if (next < tend) {
+  if (DEPTH == 2 && tile < tend) {
+    const uint64_t lb = tile + G8 < tend ? nbase : base;
 #pragma unroll
-      for (int it = 0; it < IT; ++it)
-        v[it] = ld_nt(a + (nbase | it_addr(it) | addr_t));
+    for (int it = 0; it < IT; ++it) v0[it] = ld_nt(a + (lb | it_addr(it) | addr_t));
+  }
+
+  // one tile: load tile + DEPTH into `ld`, apply the blocks, write the tile
+  // back and move tile + 1 from `fl` into LDS
+  auto tile_step = [&](float4 (&ld)[IT], float4 (&fl)[IT]) {
+    const uint64_t next = tile + G8;
+    {
+      // issued on every path (past the last tile it reads this tile again,
+      // unused): the wait for `fl` before the fill then counts the same
+      // outstanding loads on every path, and does not wait for `ld`
+      const uint64_t lb = tile + uint64_t(DEPTH) * G8 < tend ? (DEPTH == 2 ? nbase2 : nbase)
+                                                             : base;
+#pragma unroll
+      for (int it = 0; it < IT; ++it) ld[it] = ld_nt(a + (lb | it_addr(it) | addr_t));
     }
 
     if (SPLIT) {
@@ -910,21 +947,25 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
         // every block is a k = 4 block (the host pads smaller ones); one set
         // of per-lane values, read in place once a block's last chain is
         // issued; the scalars alternate between two sets
-        K4State st;
-        st.pend = 0;
-        if constexpr (K4 == 2)
-          read_k4g_b0(sa, ra, st);
-        else
-          read_k4_b0(sa, ra, st);
-        for (int b = 0; b < nb; ++b) {
-          const int bn = b + 1 < nb ? b + 1 : 0;
-          if constexpr (K4 == 2)
+        if constexpr (K4 == 2) {
+          K4GState st;
+          read_k4g_both(sa, ra, st);
+          for (int b = 0; b < nb; ++b) {
+            const int bn = b + 1 < nb ? b + 1 : 0;
             tp_block4g<NTH, TILE>(sa, ra, b == 0 ? 1.0f : -1.0f, fb + bn,
                                   OPS + bn * kTileStrideFp32, wave, sb, ra, st);
-          else
+            sa = sb;
+          }
+        } else {
+          K4State st;
+          st.pend = 0;
+          read_k4_b0(sa, ra, st);
+          for (int b = 0; b < nb; ++b) {
+            const int bn = b + 1 < nb ? b + 1 : 0;
             tp_block4p<NTH, TILE>(sa, ra, fb + bn, OPS + bn * kTileStrideFp32, wave, sb, ra,
                                   st);
-          sa = sb;
+            sa = sb;
+          }
         }
       } else {
         for (int b = 0;;) {
@@ -949,22 +990,36 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
       else
         o[it] = make_float4(re[s0], im[s0], re[s0 ^ 1], im[s0 ^ 1]);
     }
-    __syncthreads();  // the finished tile is out of LDS
+    // each thread refills the slots it has just read out: no other thread
+    // touches them in between, so no barrier is needed here
     if (next < tend) {
 #pragma unroll
       for (int it = 0; it < IT; ++it) {
         const int s0 = slot_t ^ swz(it << TBITS);
-        re[s0] = v[it].x;
-        im[s0] = v[it].y;
-        re[s0 ^ 1] = v[it].z;
-        im[s0 ^ 1] = v[it].w;
+        re[s0] = fl[it].x;
+        im[s0] = fl[it].y;
+        re[s0 ^ 1] = fl[it].z;
+        im[s0 ^ 1] = fl[it].w;
       }
     }
     __builtin_amdgcn_sched_barrier(0);  // keep the stores after the LDS writes
 #pragma unroll
     for (int it = 0; it < IT; ++it) st_nt(a + (base | it_addr(it) | addr_t), o[it]);
     __syncthreads();  // the next tile is in LDS
+    tile = next;
     base = nbase;
+    nbase = nbase2;
+    nbase2 = step_base(nbase2);
+  };
+  if constexpr (DEPTH == 2) {
+    // v0 holds tile + 1 on entry; the sets alternate
+    while (tile < tend) {
+      tile_step(v1, v0);
+      if (tile >= tend) break;
+      tile_step(v0, v1);
+    }
+  } else {
+    while (tile < tend) tile_step(v0, v0);
   }
 }
 
