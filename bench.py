@@ -119,6 +119,15 @@ def timing(hip, state, cls):
     return ms.value, n.value, b.value, f.value
 
 
+def moved(hip, state, cls):
+    """Bytes of the 128-byte lines the class's launches read and wrote."""
+    import ctypes
+    mv = ctypes.c_double()
+    if hip.qcs_hip_timing_moved(state, cls, ctypes.byref(mv)) != 0:
+        raise RuntimeError(hip.qcs_hip_last_error())
+    return mv.value
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as fh:
@@ -524,8 +533,10 @@ def bench_observe(args):
     writes the CDF (chunk sums, then one 4096-amplitude chunk per shot).  A
     measurement of qubit 7 reads the bit-7 = 0 half for p0 (and the other half
     for p1 only on outcome 1), then reads the kept half and writes the whole
-    state: 2*N*s here, where every timed call has outcome 0 (u = 0.25, and
-    the first call leaves p0 = 1).  Two gate kernels that the other
+    state: 2*N*s moved for outcome 0, 2.5*N*s for outcome 1.  Before each
+    timed measurement an untimed H on qubit 7 makes p0 = p1 = 1/2, and
+    u = 0.25 or 0.75 picks the outcome; both are reported, and their mean.
+    Two gate kernels that the other
     workloads' circuits do not reach are timed here too: the phase gate
     (N*s: half the state read and written) and, on a reference-compatible
     circuit, the out-of-place controlled gate (2*N*s)."""
@@ -563,6 +574,14 @@ def bench_observe(args):
         amp[0] *= 1.5
         ok(hip.qcs_hip_write(st, 0, 1, amp))
 
+    r2 = math.sqrt(0.5)
+    hm = (ctypes.c_double * 8)(r2, 0.0, r2, 0.0, r2, 0.0, -r2, 0.0)
+
+    def balance_q7():
+        # after a measurement qubit 7 is in a basis state: H makes both
+        # outcomes equally likely again
+        ok(hip.qcs_hip_apply_1q(st, 7, hm))
+
     # name: (call, algorithmic bytes, bytes this implementation moves, setup)
     chunk = 4096 * s  # a shot scans one 4096-amplitude chunk
     ops = {
@@ -571,7 +590,10 @@ def bench_observe(args):
                       perturb),
         "argmax": (lambda: ok(hip.qcs_hip_argmax(st, ctypes.byref(l))), N * s, N * s, None),
         "measure_qubit_7": (lambda: ok(hip.qcs_hip_measure(st, 7, 0.25, ctypes.byref(o))),
-                            2 * N * s, 2 * N * s, None),
+                            2 * N * s, 2 * N * s, balance_q7),
+        "measure_qubit_7_outcome_1": (
+            lambda: ok(hip.qcs_hip_measure(st, 7, 0.75, ctypes.byref(o))),
+            2 * N * s, 2.5 * N * s, balance_q7),
         "sample_1024_shots": (lambda: ok(hip.qcs_hip_sample(st, shots, u, idx)),
                               N * s + 2 * N * 8 + shots * n * 8, N * s + shots * chunk, None),
         "grover_round": (lambda: ok(hip.qcs_hip_grover(st, 12345, 1)), 3 * N * s, 3 * N * s,
@@ -604,6 +626,11 @@ def bench_observe(args):
             print(f"bench: observe {name} {t * 1e3:.3f} ms", file=sys.stderr, flush=True)
 
     time_ops(st, ops)
+    m0, m1 = out["measure_qubit_7"], out["measure_qubit_7_outcome_1"]
+    out["measure_qubit_7_mean_of_outcomes"] = {
+        "ms": (m0["ms"] + m1["ms"]) / 2, "algorithmic_bytes": 2 * N * s,
+        "gbs": 2 * N * s / ((m0["ms"] + m1["ms"]) / 2e3) / 1e9,
+        "moved_bytes": 2.25 * N * s}
     c.close()
     # the reference-compatible controlled gate (out of place over the whole
     # state, then a buffer swap: 2*N*s), on a compat circuit (two buffers)
@@ -1158,6 +1185,7 @@ def main():
         Tp = time.perf_counter() - t1
         m1, n1, b1, _ = timing(hip, pstate, qcs_amd.K_1Q)
         m2, n2, b2, _ = timing(hip, pstate, qcs_amd.K_CU)
+        mv1, mv2 = moved(hip, pstate, qcs_amd.K_1Q), moved(hip, pstate, qcs_amd.K_CU)
         omf, mx = fused.compare_with(plain)
         result["plain_fp64"] = {
             "ms": Tp * 1e3,
@@ -1169,6 +1197,12 @@ def main():
             "k_cu_gbs": (b2 / max(n2, 1)) / (m2 / max(n2, 1) * 1e-3) / 1e9 if n2 else None,
         }
         result["plain_fp64"]["k_1q_hbm_frac"] = (result["plain_fp64"]["k_1q_gbs"] or 0) / HBM_PEAK_GBS
+        # what HBM moved: whole 128-byte lines (a controlled gate on qubit 0,
+        # 1 or 2 reads and writes every line, twice its algorithmic bytes)
+        for key, mv, ms_, n_ in (("k_1q", mv1, m1, n1), ("k_cu", mv2, m2, n2)):
+            gbs = mv / (ms_ * 1e-3) / 1e9 if n_ else None
+            result["plain_fp64"][key + "_moved_gbs"] = gbs
+            result["plain_fp64"][key + "_moved_hbm_frac"] = (gbs or 0) / HBM_PEAK_GBS
         result["speedup_vs_plain_fp64"] = Tp / m["ms_per_step"] * 1e3
         result["one_minus_f"] = omf
         result["max_abs_err"] = mx

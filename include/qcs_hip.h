@@ -290,6 +290,12 @@ int qcs_hip_timing_enable(qcs_hip_state *s, int enable);
 /* per class: total ms, launches, algorithmic bytes, flops since enable */
 int qcs_hip_timing_read(qcs_hip_state *s, int kernel_class, double *total_ms,
                         long *launches, double *bytes, double *flops);
+/* per class: the bytes of the 128-byte lines the launches read and write
+ * since enable (HBM moves whole lines: a gate whose changed amplitudes
+ * share lines with unchanged ones, e.g. a controlled gate whose control is
+ * qubit 0, 1 or 2 on a complex128 state, moves more than its algorithmic
+ * bytes).  No counterpart in the reference (kernel timing). */
+int qcs_hip_timing_moved(qcs_hip_state *s, int kernel_class, double *moved_bytes);
 
 /* ---- batched circuits ----------------------------------------------------
  * Many independent small circuits in one persistent launch.  Each circuit

@@ -111,6 +111,7 @@ _HIP_PROTOS = {
     "qcs_hip_timing_enable": (c_int, [c_void_p, c_int]),
     "qcs_hip_timing_read": (c_int, [c_void_p, c_int, P(c_double), P(c_long),
                                     P(c_double), P(c_double)]),
+    "qcs_hip_timing_moved": (c_int, [c_void_p, c_int, P(c_double)]),
     "qcs_hip_batch_create": (c_int, [P(c_void_p), c_int]),
     "qcs_hip_batch_destroy": (None, [c_void_p]),
     "qcs_hip_batch_set_tuning": (c_int, [c_void_p, c_int, c_int]),

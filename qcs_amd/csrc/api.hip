@@ -75,11 +75,12 @@ static hipEvent_t take_event(qcs_hip_state *s) {
 }
 
 void timing_begin(qcs_hip_state *s, int cls, double bytes, double flops,
-                  TimedLaunch *tl) {
+                  TimedLaunch *tl, double moved) {
   tl->start = tl->stop = nullptr;
   tl->cls = cls;
   tl->bytes = bytes;
   tl->flops = flops;
+  tl->moved = moved < 0.0 ? bytes : moved;
   if (!s->timing) return;
   tl->start = take_event(s);
   tl->stop = take_event(s);
@@ -103,6 +104,7 @@ static int drain_events(qcs_hip_state *s) {
     s->t_launches[tl.cls] += 1;
     s->t_bytes[tl.cls] += tl.bytes;
     s->t_flops[tl.cls] += tl.flops;
+    s->t_moved[tl.cls] += tl.moved;
     s->free_events.push_back(tl.start);
     s->free_events.push_back(tl.stop);
   }
@@ -691,6 +693,7 @@ int qcs_hip_timing_enable(qcs_hip_state *s, int enable) {
     s->t_launches[c] = 0;
     s->t_bytes[c] = 0.0;
     s->t_flops[c] = 0.0;
+    s->t_moved[c] = 0.0;
   }
   s->timing = enable ? 1 : 0;
   return 0;
@@ -707,6 +710,17 @@ int qcs_hip_timing_read(qcs_hip_state *s, int kernel_class, double *total_ms,
   if (launches) *launches = s->t_launches[kernel_class];
   if (bytes) *bytes = s->t_bytes[kernel_class];
   if (flops) *flops = s->t_flops[kernel_class];
+  return 0;
+}
+
+int qcs_hip_timing_moved(qcs_hip_state *s, int kernel_class, double *moved_bytes) {
+  int rc;
+  if ((rc = check_state(s))) return rc;
+  if (kernel_class < 0 || kernel_class >= QCS_HIP_K_COUNT)
+    return fail(QCS_HIP_EINVAL, "unknown kernel class %d", kernel_class);
+  if (!moved_bytes) return fail(QCS_HIP_EINVAL, "null output");
+  if ((rc = drain_events(s))) return rc;
+  *moved_bytes = s->t_moved[kernel_class];
   return 0;
 }
 

@@ -267,6 +267,15 @@ static bool lane_pairs(int t, size_t amp_bytes) { return (size_t(1) << t) * amp_
 // shorter than a 64-byte line
 static bool short_runs(int b, size_t amp_bytes) { return (size_t(1) << b) * amp_bytes < 64; }
 
+// HBM bytes a kernel moves when it reads and writes the amplitudes whose
+// index bits in `fixed` are all 1: HBM moves whole 128-byte lines, so only
+// the fixed bits above a line's index bits thin out the lines touched
+static double line_bytes(double n, size_t amp_bytes, uint64_t fixed) {
+  const int in_line = __builtin_ctzll(uint64_t(128) / amp_bytes);
+  return 2.0 * n * double(amp_bytes) /
+         double(uint64_t(1) << __builtin_popcountll(fixed >> in_line));
+}
+
 // Compat controlled gate, out of place over all N (src/q_gates.c:277-292 as
 // written): out[i] = m00*v[i] + m01*v[i|t] where bit c = 1 and bit t = 0,
 // out[i] = v[i] everywhere else (iteration i|t overwrites the partner).  One
@@ -366,7 +375,8 @@ static int do_1q(qcs_hip_state *s, int t, const double md[8]) {
   TimedLaunch tl;
   if (!compat && is_phase1(md)) {
     const uint64_t nhalf = s->size / 2;
-    timing_begin(s, QCS_HIP_K_1Q, N * sb, 3.0 * N, &tl);
+    timing_begin(s, QCS_HIP_K_1Q, N * sb, 3.0 * N, &tl,
+                 line_bytes(N, sizeof(A), uint64_t(1) << t));
     hipLaunchKernelGGL(k_1q_phase<T>, dim3(grid_for(nhalf)), dim3(kThreads), 0,
                        s->stream, cur, nhalf, t, m.v[6], m.v[7]);
   } else if (!compat && lane_pairs(t, sizeof(A))) {
@@ -409,10 +419,11 @@ static int do_cu(qcs_hip_state *s, int c, int t, const double md[8]) {
     const uint64_t nq = s->size / 4;
     const dim3 g(grid_for(nq)), b(kThreads);
     const bool x = is_x(md), ph = !x && is_phase1(md);
+    const double mv = line_bytes(N, sizeof(A), ph ? (cb | tb) : cb);
     // the lowest bit fixed in the changed amplitudes: the control, or for a
     // controlled phase the lower of the two
     if (short_runs(ph ? lo : c, sizeof(A))) {
-      timing_begin(s, QCS_HIP_K_CU, (ph ? 0.5 : 1.0) * N * sb, 0.0, &tl);
+      timing_begin(s, QCS_HIP_K_CU, (ph ? 0.5 : 1.0) * N * sb, 0.0, &tl, mv);
       const uint64_t np = s->size / 2;
       const dim3 gp(grid_for(np));
       if (!ph && lane_pairs(t, sizeof(A))) {
@@ -432,15 +443,15 @@ static int do_cu(qcs_hip_state *s, int c, int t, const double md[8]) {
         hipLaunchKernelGGL((k_pair_sweep<2, false, T>), gp, b, 0, s->stream, cur, np, hi,
                            uint64_t(1) << lo, m);
     } else if (x) {
-      timing_begin(s, QCS_HIP_K_CU, N * sb, 0.0, &tl);
+      timing_begin(s, QCS_HIP_K_CU, N * sb, 0.0, &tl, mv);
       hipLaunchKernelGGL((k_cu<1, T>), g, b, 0, s->stream, cur, nq, lo, hi, cb,
                          tb, m);
     } else if (ph) {
-      timing_begin(s, QCS_HIP_K_CU, 0.5 * N * sb, 1.5 * N, &tl);
+      timing_begin(s, QCS_HIP_K_CU, 0.5 * N * sb, 1.5 * N, &tl, mv);
       hipLaunchKernelGGL((k_cu<2, T>), g, b, 0, s->stream, cur, nq, lo, hi,
                          cb | tb, uint64_t(0), m);
     } else {
-      timing_begin(s, QCS_HIP_K_CU, N * sb, 7.0 * N, &tl);
+      timing_begin(s, QCS_HIP_K_CU, N * sb, 7.0 * N, &tl, mv);
       hipLaunchKernelGGL((k_cu<0, T>), g, b, 0, s->stream, cur, nq, lo, hi, cb,
                          tb, m);
     }
@@ -464,9 +475,12 @@ int launch_swap(qcs_hip_state *s, int a, int b) {
   const int lo = a < b ? a : b, hi = a < b ? b : a;
   const uint64_t nq = s->size / 4;
   TimedLaunch tl;
-  timing_begin(s, QCS_HIP_K_CU, double(s->size) * double(s->amp_bytes), 0.0,
-               &tl);
   const bool sweep = short_runs(lo, size_t(s->amp_bytes));
+  // the exchanged amplitudes have bit lo = 1 or bit hi = 1 but not both:
+  // every line holds some when lo is inside a line
+  const bool lo_in_line = (size_t(1) << lo) * size_t(s->amp_bytes) < 128;
+  timing_begin(s, QCS_HIP_K_CU, double(s->size) * double(s->amp_bytes), 0.0, &tl,
+               (lo_in_line ? 2.0 : 1.0) * double(s->size) * double(s->amp_bytes));
   const uint64_t np = s->size / 2;
   if (s->precision == QCS_HIP_C128) {
     if (sweep)

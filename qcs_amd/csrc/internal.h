@@ -27,8 +27,9 @@ struct TimedLaunch {
   hipEvent_t start;
   hipEvent_t stop;
   int cls;
-  double bytes;
+  double bytes;   // algorithmic: the amplitudes the operation reads and writes
   double flops;
+  double moved;   // the bytes of every 128-byte line those amplitudes lie in
 };
 
 }  // namespace qcs
@@ -82,6 +83,7 @@ struct qcs_hip_state {
   long t_launches[QCS_HIP_K_COUNT] = {0};
   double t_bytes[QCS_HIP_K_COUNT] = {0};
   double t_flops[QCS_HIP_K_COUNT] = {0};
+  double t_moved[QCS_HIP_K_COUNT] = {0};
 };
 
 namespace qcs {
@@ -100,9 +102,10 @@ int check_hip(hipError_t e, const char *what);
 int validate_blocks(int n, const qcs_hip_block *blocks, int num_blocks,
                     const qcs_hip_gate *gates, int num_gates);
 
-// launch bracketing for the kernel timers
+// launch bracketing for the kernel timers; `moved` (default: `bytes`) for a
+// kernel whose amplitudes share 128-byte lines with ones it does not change
 void timing_begin(qcs_hip_state *s, int cls, double bytes, double flops,
-                  TimedLaunch *tl);
+                  TimedLaunch *tl, double moved = -1.0);
 int timing_end(qcs_hip_state *s, TimedLaunch *tl);
 // after a launch: check for a launch error
 int launch_check(const char *what);

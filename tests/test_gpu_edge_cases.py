@@ -178,3 +178,39 @@ def test_device_gates_on_every_bit_pair(qlib, options, tol):
                     assert hip.qcs_hip_apply_cu(st, ctl, t, mm) == 0
                     got, want = c.amplitudes(), HP.apply_cu(psi, n, ctl, t, m)
                     assert np.max(np.abs(got - want)) < tol, ("cu", ctl, t)
+
+
+def test_timing_counts_algorithmic_and_moved_bytes(qlib):
+    """qcs_hip_timing_read counts the amplitudes a gate changes (read and
+    written); qcs_hip_timing_moved the 128-byte lines they lie in.  On a
+    complex128 state (8 amplitudes a line) a CNOT whose control is qubit 0
+    or 2 changes half of every line and moves the whole state twice; one
+    whose control is qubit 10 moves only its algorithmic bytes; a phase gate
+    on qubit 1 or 9 likewise; a SWAP of qubits 1 and 8 touches every line."""
+    hip, _ = qlib
+    n = 16
+    N, s = float(1 << n), 16.0
+    x = (ctypes.c_double * 8)(0.0, 0.0, 1.0, 0.0, 1.0, 0.0, 0.0, 0.0)
+    ph = (ctypes.c_double * 8)(1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 1.0)
+    with qcs_amd.Circuit(n) as c:
+        st = c.device_state
+        cases = [(qcs_amd.K_CU, lambda: hip.qcs_hip_apply_cu(st, 0, 5, x), N * s, 2 * N * s),
+                 (qcs_amd.K_CU, lambda: hip.qcs_hip_apply_cu(st, 2, 5, x), N * s, 2 * N * s),
+                 (qcs_amd.K_CU, lambda: hip.qcs_hip_apply_cu(st, 10, 5, x), N * s, N * s),
+                 (qcs_amd.K_CU, lambda: hip.qcs_hip_apply_swap(st, 1, 8), N * s, 2 * N * s),
+                 (qcs_amd.K_CU, lambda: hip.qcs_hip_apply_swap(st, 4, 8), N * s, N * s),
+                 (qcs_amd.K_1Q, lambda: hip.qcs_hip_apply_1q(st, 1, ph), N * s, 2 * N * s),
+                 (qcs_amd.K_1Q, lambda: hip.qcs_hip_apply_1q(st, 9, ph), N * s, N * s)]
+        for cls, fn, algo, mv in cases:
+            assert hip.qcs_hip_timing_enable(st, 1) == 0
+            assert fn() == 0, hip.qcs_hip_last_error()
+            ms, nl, b, f = (ctypes.c_double(), ctypes.c_long(), ctypes.c_double(),
+                            ctypes.c_double())
+            got = ctypes.c_double()
+            assert hip.qcs_hip_timing_read(st, cls, ctypes.byref(ms), ctypes.byref(nl),
+                                           ctypes.byref(b), ctypes.byref(f)) == 0
+            assert hip.qcs_hip_timing_moved(st, cls, ctypes.byref(got)) == 0
+            assert nl.value == 1 and b.value == algo and got.value == mv, (cls, b.value, got.value)
+        assert hip.qcs_hip_timing_moved(st, qcs_amd.K_COUNT, ctypes.byref(got)) == -1
+        assert hip.qcs_hip_timing_moved(st, qcs_amd.K_CU, None) == -1
+        assert hip.qcs_hip_timing_enable(st, 0) == 0
