@@ -1171,7 +1171,11 @@ def main():
         with open(traffic_file) as fh:
             t = json.load(fh).get("k_tile_program" if tiled else "k_fused_mfma")
         if t:
+            # measured by rocprofv3 in a separate run of this workload, not live
             result["roofline"]["traffic"] = t.get("bytes_per_launch")
+            result["roofline"]["traffic_source"] = (
+                "profiles/traffic.json: rocprofv3 FETCH_SIZE and WRITE_SIZE of a separate run "
+                "of this workload, scaled by a copy of known size")
 
     plain = None
     if not args.no_plain:
