@@ -823,8 +823,6 @@ __device__ void run_c64(const BCirc c, const BPass *__restrict__ passes,
   }
 }
 
-// The persistent kernel.  `queue` starts at 0 (k_batch_reset); each
-// work-group takes circuits until the counter passes nc.
 // The persistent kernel.  `queue[0]` starts at 0 (k_batch_reset); each
 // work-group takes queue items until the counter passes nitems.  The parts
 // of a shared circuit are consecutive items, at most half the grid, and
