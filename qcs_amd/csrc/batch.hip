@@ -385,10 +385,35 @@ __device__ void tile_group(double2 *L, int tb, const BGroup *__restrict__ gp,
   }
 }
 
-// complex128 tile <-> HBM: element e <-> G[base | deposit(e)].  Loads go
-// out kIoU per thread at a time (8 x 16 bytes in flight per lane), so one
-// work-group keeps 64 KiB of a tile in flight.
-constexpr int kIoU = 8;
+// complex128 tile <-> HBM: element e <-> G[base | deposit(e)].  A thread
+// moves the elements e = tid (mod NTH), through LDS slot dswz(e) (the same
+// slots in both directions, so a store phase followed by the next tile's
+// load needs no barrier between them).  Loads go out up to 16 per thread at
+// a time: a 13-bit tile (128 KiB) in one round trip at 512 threads (the
+// register-group values are dead here, so the registers are free).
+template <int U, int NTH, bool LOAD>
+__device__ __forceinline__ int io_rounds_c128(double2 *L, double2 *G, uint64_t dt,
+                                              const int *sb, int tb, int e0) {
+  constexpr int LT = __builtin_ctz(NTH);
+  const int tid = threadIdx.x, tile = 1 << tb;
+  for (; e0 + U * NTH <= tile; e0 += U * NTH) {
+    uint64_t a[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      a[u] = dt | deposit(uint32_t(e0 + u * NTH + tid), sb, LT, tb);
+    if (LOAD) {
+      double2 v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) v[u] = G[a[u]];
+#pragma unroll
+      for (int u = 0; u < U; ++u) L[dswz(e0 + u * NTH + tid)] = v[u];
+    } else {
+#pragma unroll
+      for (int u = 0; u < U; ++u) G[a[u]] = L[dswz(e0 + u * NTH + tid)];
+    }
+  }
+  return e0;
+}
 
 template <int NTH, bool LOAD>
 __device__ void tile_io_c128(double2 *L, double2 *G, uint64_t base,
@@ -396,23 +421,8 @@ __device__ void tile_io_c128(double2 *L, double2 *G, uint64_t base,
   constexpr int LT = __builtin_ctz(NTH);
   const int tid = threadIdx.x, tile = 1 << tb;
   const uint64_t dt = base | deposit(uint32_t(tid), sb, 0, tb < LT ? tb : LT);
-  int e0 = 0;
-  for (; e0 + kIoU * NTH <= tile; e0 += kIoU * NTH) {
-    uint64_t a[kIoU];
-#pragma unroll
-    for (int u = 0; u < kIoU; ++u)
-      a[u] = dt | deposit(uint32_t(e0 + u * NTH + tid), sb, LT, tb);
-    if (LOAD) {
-      double2 v[kIoU];
-#pragma unroll
-      for (int u = 0; u < kIoU; ++u) v[u] = G[a[u]];
-#pragma unroll
-      for (int u = 0; u < kIoU; ++u) L[dswz(e0 + u * NTH + tid)] = v[u];
-    } else {
-#pragma unroll
-      for (int u = 0; u < kIoU; ++u) G[a[u]] = L[dswz(e0 + u * NTH + tid)];
-    }
-  }
+  int e0 = io_rounds_c128<16, NTH, LOAD>(L, G, dt, sb, tb, 0);
+  e0 = io_rounds_c128<4, NTH, LOAD>(L, G, dt, sb, tb, e0);
   for (int e = e0 + tid; e < tile; e += NTH) {
     const uint64_t a = dt | deposit(uint32_t(e), sb, LT, tb);
     if (LOAD) L[dswz(e)] = G[a];
@@ -527,6 +537,39 @@ __device__ __forceinline__ void pair_io_c64(char *Lb, float2 *G, uint64_t a,
   }
 }
 
+// A thread moves the pairs p = tid (mod NTH) through the same LDS slots in
+// both directions; loads go out up to 16 per thread at a time (a 14-bit
+// tile in one round trip at 512 threads).
+template <int U, int NTH, bool LOAD>
+__device__ __forceinline__ int io_rounds_c64(char *Lb, float2 *G, uint64_t dt,
+                                             const int *sb, int tb, int p0) {
+  constexpr int LT = __builtin_ctz(NTH);
+  const int tid = threadIdx.x, npair = 1 << (tb - 1);
+  for (; p0 + U * NTH <= npair; p0 += U * NTH) {
+    uint64_t a[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      a[u] = dt | deposit(uint32_t(p0 + u * NTH + tid) << 1, sb, 1 + LT, tb);
+    if (LOAD) {
+      bf4v v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) v[u] = *reinterpret_cast<const bf4v *>(G + a[u]);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int pp = p0 + u * NTH + tid;
+        float2 *s0 = reinterpret_cast<float2 *>(Lb + 8 * bswz(2 * pp));
+        float2 *s1 = reinterpret_cast<float2 *>(Lb + 8 * (bswz(2 * pp) ^ 1));
+        *s0 = make_float2(v[u].x, v[u].y);
+        *s1 = make_float2(v[u].z, v[u].w);
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < U; ++u) pair_io_c64<NTH, false>(Lb, G, a[u], p0 + u * NTH + tid);
+    }
+  }
+  return p0;
+}
+
 template <int NTH, bool LOAD>
 __device__ void tile_io_c64(char *Lb, float2 *G, uint64_t base, const int *sb,
                             int tb) {
@@ -534,34 +577,8 @@ __device__ void tile_io_c64(char *Lb, float2 *G, uint64_t base, const int *sb,
   const int tid = threadIdx.x, npair = 1 << (tb - 1);
   const int jt = tb - 1 < LT ? tb - 1 : LT;  // tile bits 1..jt from tid
   const uint64_t dt = base | deposit(uint32_t(tid) << 1, sb, 1, 1 + jt);
-  int p0 = 0;
-  for (; p0 + 4 * NTH <= npair; p0 += 4 * NTH) {
-    const int pa = p0 + tid, pb = pa + NTH, pc = pb + NTH, pd = pc + NTH;
-    const uint64_t aa = dt | deposit(uint32_t(pa) << 1, sb, 1 + LT, tb);
-    const uint64_t ab = dt | deposit(uint32_t(pb) << 1, sb, 1 + LT, tb);
-    const uint64_t ac = dt | deposit(uint32_t(pc) << 1, sb, 1 + LT, tb);
-    const uint64_t ad = dt | deposit(uint32_t(pd) << 1, sb, 1 + LT, tb);
-    if (LOAD) {
-      const bf4v va = *reinterpret_cast<const bf4v *>(G + aa);
-      const bf4v vb = *reinterpret_cast<const bf4v *>(G + ab);
-      const bf4v vc = *reinterpret_cast<const bf4v *>(G + ac);
-      const bf4v vd = *reinterpret_cast<const bf4v *>(G + ad);
-      const bf4v vv[4] = {va, vb, vc, vd};
-      const int pp[4] = {pa, pb, pc, pd};
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        float2 *s0 = reinterpret_cast<float2 *>(Lb + 8 * bswz(2 * pp[u]));
-        float2 *s1 = reinterpret_cast<float2 *>(Lb + 8 * (bswz(2 * pp[u]) ^ 1));
-        *s0 = make_float2(vv[u].x, vv[u].y);
-        *s1 = make_float2(vv[u].z, vv[u].w);
-      }
-    } else {
-      pair_io_c64<NTH, false>(Lb, G, aa, pa);
-      pair_io_c64<NTH, false>(Lb, G, ab, pb);
-      pair_io_c64<NTH, false>(Lb, G, ac, pc);
-      pair_io_c64<NTH, false>(Lb, G, ad, pd);
-    }
-  }
+  int p0 = io_rounds_c64<16, NTH, LOAD>(Lb, G, dt, sb, tb, 0);
+  p0 = io_rounds_c64<4, NTH, LOAD>(Lb, G, dt, sb, tb, p0);
   for (int p = p0 + tid; p < npair; p += NTH)
     pair_io_c64<NTH, LOAD>(Lb, G, dt | deposit(uint32_t(p) << 1, sb, 1 + LT, tb), p);
 }
@@ -719,7 +736,10 @@ __device__ void run_c128(const BCirc c, int part, int nparts, unsigned *sync,
       }
       if (!resident) {
         tile_io_c128<NTH, false>(L, G, base, ps.sbits, tb);
-        __syncthreads();
+        // the next tile of this pass loads into the slots each thread has
+        // just stored from; the first pass's tiles are set up by element
+        // instead, and the next pass reads what other threads stored
+        if (p == 0 || t + uint64_t(nparts) >= ntiles) __syncthreads();
       }
     }
   }
@@ -777,7 +797,8 @@ __device__ void run_c64(const BCirc c, const BPass *__restrict__ passes,
       }
       if (!resident) {
         tile_io_c64<NTH, false>(lds, G, base, ps.sbits, tb);
-        __syncthreads();
+        // as in run_c128: within a pass each thread reloads its own slots
+        if (p == 0 || t + 1 >= ntiles) __syncthreads();
       }
     }
   }
