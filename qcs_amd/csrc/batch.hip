@@ -145,20 +145,52 @@ __device__ __forceinline__ int bins0(int x, int b) {
   return ((x >> b) << (b + 1)) | (x & ((1 << b) - 1));
 }
 
+// A pass's tile positions -> index bits, ascending, held in registers (the
+// same in every thread) so that the address arithmetic of a tile reads no
+// memory; entries at j >= tb are never used.
+struct TileBits {
+  int sb[kBMaxTb];
+};
+
+__device__ __forceinline__ TileBits load_tile_bits(const int *src) {
+  TileBits b;
+#pragma unroll
+  for (int j = 0; j < kBMaxTb; ++j) b.sb[j] = src[j];
+  return b;
+}
+
 // first index of tile t: t's bits spread over the index bits outside S
-__device__ __forceinline__ uint64_t tile_base(uint64_t t, const int *sb,
-                                              int tb) {
-  for (int j = 0; j < tb; ++j) t = insert0(t, sb[j]);
+__device__ __forceinline__ uint64_t tile_base(uint64_t t, const TileBits &b, int tb) {
+#pragma unroll
+  for (int j = 0; j < kBMaxTb; ++j)
+    if (j < tb) t = insert0(t, b.sb[j]);
   return t;
 }
 
+// Index bits outside the tile, of an n-qubit state.  Tile first indices
+// are t's bits spread over these, so the first index of tile t + d is
+// ((base(t) | ~M) + base(d)) & M: the carries run through the bits of M.
+__device__ __forceinline__ uint64_t outside_mask(const TileBits &b, int tb, int n) {
+  uint64_t m = (uint64_t(1) << n) - 1;
+#pragma unroll
+  for (int j = 0; j < kBMaxTb; ++j)
+    if (j < tb) m &= ~(uint64_t(1) << b.sb[j]);
+  return m;
+}
+
+__device__ __forceinline__ uint64_t masked_add(uint64_t x, uint64_t y, uint64_t m) {
+  return ((x | ~m) + y) & m;
+}
+
 // index bits of tile element e: bit j of e goes to index bit sb[j], for
-// j in [j0, tb)
-__device__ __forceinline__ uint64_t deposit(uint32_t e, const int *sb, int j0,
-                                            int tb) {
+// j in [J0, min(J1, tb)).  With e the same in every thread (the element
+// bits above the thread bits) this is scalar arithmetic.
+template <int J0, int J1>
+__device__ __forceinline__ uint64_t deposit(uint32_t e, const TileBits &b, int tb) {
   uint64_t a = 0;
-  for (int j = j0; j < tb; ++j)
-    if ((e >> j) & 1u) a |= uint64_t(1) << sb[j];
+#pragma unroll
+  for (int j = J0; j < J1; ++j)
+    if (j < tb && ((e >> j) & 1u)) a |= uint64_t(1) << b.sb[j];
   return a;
 }
 
@@ -393,14 +425,27 @@ __device__ void tile_group(double2 *L, int tb, const BGroup *__restrict__ gp,
 // register-group values are dead here, so the registers are free).
 template <int U, int NTH, bool LOAD>
 __device__ __forceinline__ int io_rounds_c128(double2 *L, double2 *G, uint64_t dt,
-                                              const int *sb, int tb, int e0) {
+                                              const TileBits &b, int tb, int e0) {
   constexpr int LT = __builtin_ctz(NTH);
   const int tid = threadIdx.x, tile = 1 << tb;
+  constexpr int LU = __builtin_ctz(U);
+  static_assert(LT + LU <= kBMaxTb, "tile bit index in range");
+  // element e0 + u * NTH + tid: bits below LT are tid's (in dt), bits LT ..
+  // LT + LU - 1 are u's, the bits above are e0's (a multiple of U * NTH)
+  uint64_t bit[LU];
+#pragma unroll
+  for (int k = 0; k < LU; ++k) bit[k] = uint64_t(1) << (b.sb[LT + k] & 63);
   for (; e0 + U * NTH <= tile; e0 += U * NTH) {
+    const uint64_t d0 = dt | deposit<LT + LU, kBMaxTb>(uint32_t(e0), b, tb);
     uint64_t a[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u)
-      a[u] = dt | deposit(uint32_t(e0 + u * NTH + tid), sb, LT, tb);
+    for (int u = 0; u < U; ++u) {
+      uint64_t du = 0;
+#pragma unroll
+      for (int k = 0; k < LU; ++k)
+        if ((u >> k) & 1) du |= bit[k];
+      a[u] = d0 | du;
+    }
     if (LOAD) {
       double2 v[U];
 #pragma unroll
@@ -416,17 +461,19 @@ __device__ __forceinline__ int io_rounds_c128(double2 *L, double2 *G, uint64_t d
 }
 
 template <int NTH, bool LOAD>
-__device__ void tile_io_c128(double2 *L, double2 *G, uint64_t base,
-                             const int *sb, int tb) {
+__device__ void tile_io_c128(double2 *L, double2 *G, uint64_t base, const TileBits &b, int tb) {
   constexpr int LT = __builtin_ctz(NTH);
   const int tid = threadIdx.x, tile = 1 << tb;
-  const uint64_t dt = base | deposit(uint32_t(tid), sb, 0, tb < LT ? tb : LT);
-  int e0 = io_rounds_c128<16, NTH, LOAD>(L, G, dt, sb, tb, 0);
-  e0 = io_rounds_c128<4, NTH, LOAD>(L, G, dt, sb, tb, e0);
-  for (int e = e0 + tid; e < tile; e += NTH) {
-    const uint64_t a = dt | deposit(uint32_t(e), sb, LT, tb);
-    if (LOAD) L[dswz(e)] = G[a];
-    else G[a] = L[dswz(e)];
+  const uint64_t dt = base | deposit<0, LT>(uint32_t(tid), b, tb);
+  int e0 = io_rounds_c128<16, NTH, LOAD>(L, G, dt, b, tb, 0);
+  e0 = io_rounds_c128<4, NTH, LOAD>(L, G, dt, b, tb, e0);
+  for (; e0 < tile; e0 += NTH) {
+    const int e = e0 + tid;
+    if (e < tile) {
+      const uint64_t a = dt | deposit<LT, kBMaxTb>(uint32_t(e0), b, tb);
+      if (LOAD) L[dswz(e)] = G[a];
+      else G[a] = L[dswz(e)];
+    }
   }
 }
 
@@ -542,14 +589,28 @@ __device__ __forceinline__ void pair_io_c64(char *Lb, float2 *G, uint64_t a,
 // tile in one round trip at 512 threads).
 template <int U, int NTH, bool LOAD>
 __device__ __forceinline__ int io_rounds_c64(char *Lb, float2 *G, uint64_t dt,
-                                             const int *sb, int tb, int p0) {
+                                             const TileBits &b, int tb, int p0) {
   constexpr int LT = __builtin_ctz(NTH);
   const int tid = threadIdx.x, npair = 1 << (tb - 1);
+  constexpr int LU = __builtin_ctz(U);
+  static_assert(1 + LT + LU <= kBMaxTb, "tile bit index in range");
+  // pair p0 + u * NTH + tid holds elements with bits 1 .. LT from tid (in
+  // dt), bits LT + 1 .. LT + LU from u, the bits above from p0 (a multiple
+  // of U * NTH)
+  uint64_t bit[LU];
+#pragma unroll
+  for (int k = 0; k < LU; ++k) bit[k] = uint64_t(1) << (b.sb[1 + LT + k] & 63);
   for (; p0 + U * NTH <= npair; p0 += U * NTH) {
+    const uint64_t d0 = dt | deposit<1 + LT + LU, kBMaxTb>(uint32_t(p0) << 1, b, tb);
     uint64_t a[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u)
-      a[u] = dt | deposit(uint32_t(p0 + u * NTH + tid) << 1, sb, 1 + LT, tb);
+    for (int u = 0; u < U; ++u) {
+      uint64_t du = 0;
+#pragma unroll
+      for (int k = 0; k < LU; ++k)
+        if ((u >> k) & 1) du |= bit[k];
+      a[u] = d0 | du;
+    }
     if (LOAD) {
       bf4v v[U];
 #pragma unroll
@@ -571,16 +632,18 @@ __device__ __forceinline__ int io_rounds_c64(char *Lb, float2 *G, uint64_t dt,
 }
 
 template <int NTH, bool LOAD>
-__device__ void tile_io_c64(char *Lb, float2 *G, uint64_t base, const int *sb,
-                            int tb) {
+__device__ void tile_io_c64(char *Lb, float2 *G, uint64_t base, const TileBits &b, int tb) {
   constexpr int LT = __builtin_ctz(NTH);
   const int tid = threadIdx.x, npair = 1 << (tb - 1);
-  const int jt = tb - 1 < LT ? tb - 1 : LT;  // tile bits 1..jt from tid
-  const uint64_t dt = base | deposit(uint32_t(tid) << 1, sb, 1, 1 + jt);
-  int p0 = io_rounds_c64<16, NTH, LOAD>(Lb, G, dt, sb, tb, 0);
-  p0 = io_rounds_c64<4, NTH, LOAD>(Lb, G, dt, sb, tb, p0);
-  for (int p = p0 + tid; p < npair; p += NTH)
-    pair_io_c64<NTH, LOAD>(Lb, G, dt | deposit(uint32_t(p) << 1, sb, 1 + LT, tb), p);
+  // tile bits 1..LT from tid (those below tb)
+  const uint64_t dt = base | deposit<1, 1 + LT>(uint32_t(tid) << 1, b, tb);
+  int p0 = io_rounds_c64<16, NTH, LOAD>(Lb, G, dt, b, tb, 0);
+  p0 = io_rounds_c64<4, NTH, LOAD>(Lb, G, dt, b, tb, p0);
+  for (; p0 < npair; p0 += NTH) {
+    const int p = p0 + tid;
+    if (p < npair)
+      pair_io_c64<NTH, LOAD>(Lb, G, dt | deposit<1 + LT, kBMaxTb>(uint32_t(p0) << 1, b, tb), p);
+  }
 }
 
 // ---- FP64 norm and sampling ------------------------------------------------------
@@ -712,17 +775,21 @@ __device__ void run_c128(const BCirc c, int part, int nparts, unsigned *sync,
   const uint64_t ntiles = uint64_t(1) << (n - tb);
   for (int p = 0; p < c.npass; ++p) {
     const BPass &ps = passes[c.first_pass + p];
+    const TileBits tbits = load_tile_bits(ps.sbits);
     // a pass reads tiles the previous pass wrote, possibly in another part
     if (nparts > 1 && p > 0 && !parts_barrier(sync, unsigned(p) * unsigned(nparts), err))
       return;
-    for (uint64_t t = uint64_t(part); t < ntiles; t += uint64_t(nparts)) {
-      const uint64_t base = tile_base(t, ps.sbits, tb);
+    const uint64_t outside = outside_mask(tbits, tb, n);
+    const uint64_t bstep = tile_base(uint64_t(nparts), tbits, tb);
+    uint64_t base = tile_base(uint64_t(part), tbits, tb);
+    for (uint64_t t = uint64_t(part); t < ntiles;
+         t += uint64_t(nparts), base = masked_add(base, bstep, outside)) {
       if (p == 0 || resident) {
         if (p == 0)
           for (int e = tid; e < tile; e += NTH)
             L[e] = make_double2((t == 0 && e == 0) ? 1.0 : 0.0, 0.0);
       } else {
-        tile_io_c128<NTH, true>(L, G, base, ps.sbits, tb);
+        tile_io_c128<NTH, true>(L, G, base, tbits, tb);
       }
       __syncthreads();
       if (p > 0 || t == 0) {
@@ -735,7 +802,7 @@ __device__ void run_c128(const BCirc c, int part, int nparts, unsigned *sync,
         }
       }
       if (!resident) {
-        tile_io_c128<NTH, false>(L, G, base, ps.sbits, tb);
+        tile_io_c128<NTH, false>(L, G, base, tbits, tb);
         // the next tile of this pass loads into the slots each thread has
         // just stored from; the first pass's tiles are set up by element
         // instead, and the next pass reads what other threads stored
@@ -779,14 +846,17 @@ __device__ void run_c64(const BCirc c, const BPass *__restrict__ passes,
   // tiles of fewer than 32 elements are not swizzled (bswz needs tb >= 5)
   for (int p = 0; p < c.npass; ++p) {
     const BPass &ps = passes[c.first_pass + p];
-    for (uint64_t t = 0; t < ntiles; ++t) {
-      const uint64_t base = tile_base(t, ps.sbits, tb);
+    const TileBits tbits = load_tile_bits(ps.sbits);
+    const uint64_t outside = outside_mask(tbits, tb, n);
+    const uint64_t bstep = outside & (~outside + 1);  // tile 1
+    uint64_t base = 0;
+    for (uint64_t t = 0; t < ntiles; ++t, base = masked_add(base, bstep, outside)) {
       if (p == 0 || resident) {
         if (p == 0)
           for (int e = tid; e < tile; e += NTH)
             L[e] = make_float2((t == 0 && e == 0) ? 1.0f : 0.0f, 0.0f);
       } else {
-        tile_io_c64<NTH, true>(lds, G, base, ps.sbits, tb);
+        tile_io_c64<NTH, true>(lds, G, base, tbits, tb);
       }
       __syncthreads();
       if (p > 0 || t == 0) {
@@ -796,7 +866,7 @@ __device__ void run_c64(const BCirc c, const BPass *__restrict__ passes,
         }
       }
       if (!resident) {
-        tile_io_c64<NTH, false>(lds, G, base, ps.sbits, tb);
+        tile_io_c64<NTH, false>(lds, G, base, tbits, tb);
         // as in run_c128: within a pass each thread reloads its own slots
         if (p == 0 || t + 1 >= ntiles) __syncthreads();
       }
