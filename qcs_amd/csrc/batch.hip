@@ -276,31 +276,62 @@ __device__ __forceinline__ void r_1q(double (&vr)[kGA], double (&vi)[kGA], const
   for (int j = 0; j < kGA; ++j) {
     if (j & (1 << Q)) continue;
     const int k = j | (1 << Q);
-    const double2 v0 = make_double2(vr[j], vi[j]), v1 = make_double2(vr[k], vi[k]);
-    double2 o0, o1;
+    const double x0 = vr[j], y0 = vi[j], x1 = vr[k], y1 = vi[k];
+    // the products and sums of cmul / row2 (gates.hip), every product of
+    // the pair before any sum: each result is then defined after the last
+    // use of the inputs and takes an input's register, so every gate
+    // variant leaves each amplitude in the register it came in (no copies
+    // at the join after the dispatch switch)
     if constexpr (CLS == kOp1Dense) {
-      row2(m[0], m[1], m[2], m[3], v0.x, v0.y, v1.x, v1.y, o0.x, o0.y);
-      row2(m[4], m[5], m[6], m[7], v0.x, v0.y, v1.x, v1.y, o1.x, o1.y);
+      const double a0 = m[0] * x0, a1 = m[1] * y0, a2 = m[0] * y0, a3 = m[1] * x0;
+      const double b0 = m[4] * x0, b1 = m[5] * y0, b2 = m[4] * y0, b3 = m[5] * x0;
+      const double c0 = m[2] * x1, c1 = m[3] * y1, c2 = m[2] * y1, c3 = m[3] * x1;
+      const double d0 = m[6] * x1, d1 = m[7] * y1, d2 = m[6] * y1, d3 = m[7] * x1;
+      vr[j] = (a0 - a1) + (c0 - c1);
+      vi[j] = (a2 + a3) + (c2 + c3);
+      vr[k] = (b0 - b1) + (d0 - d1);
+      vi[k] = (b2 + b3) + (d2 + d3);
     } else if constexpr (CLS == kOp1Phase) {
-      o0 = v0;
-      cmul(m[6], m[7], v1.x, v1.y, o1.x, o1.y);
+      const double p0 = m[6] * x1, p1 = m[7] * y1, p2 = m[6] * y1, p3 = m[7] * x1;
+      vr[k] = p0 - p1;
+      vi[k] = p2 + p3;
     } else if constexpr (CLS == kOp1Diag) {
-      cmul(m[0], m[1], v0.x, v0.y, o0.x, o0.y);
-      cmul(m[6], m[7], v1.x, v1.y, o1.x, o1.y);
+      const double a0 = m[0] * x0, a1 = m[1] * y0, a2 = m[0] * y0, a3 = m[1] * x0;
+      const double d0 = m[6] * x1, d1 = m[7] * y1, d2 = m[6] * y1, d3 = m[7] * x1;
+      vr[j] = a0 - a1;
+      vi[j] = a2 + a3;
+      vr[k] = d0 - d1;
+      vi[k] = d2 + d3;
     } else if constexpr (CLS == kOp1Real) {
-      o0.x = m[0] * v0.x + m[2] * v1.x;
-      o0.y = m[0] * v0.y + m[2] * v1.y;
-      o1.x = m[4] * v0.x + m[6] * v1.x;
-      o1.y = m[4] * v0.y + m[6] * v1.y;
+      const double a0 = m[0] * x0, a1 = m[0] * y0, b0 = m[4] * x0, b1 = m[4] * y0;
+      const double c0 = m[2] * x1, c1 = m[2] * y1, d0 = m[6] * x1, d1 = m[6] * y1;
+      vr[j] = a0 + c0;
+      vi[j] = a1 + c1;
+      vr[k] = b0 + d0;
+      vi[k] = b1 + d1;
     } else {  // kOp1Anti
-      cmul(m[2], m[3], v1.x, v1.y, o0.x, o0.y);
-      cmul(m[4], m[5], v0.x, v0.y, o1.x, o1.y);
+      const double c0 = m[2] * x1, c1 = m[3] * y1, c2 = m[2] * y1, c3 = m[3] * x1;
+      const double b0 = m[4] * x0, b1 = m[5] * y0, b2 = m[4] * y0, b3 = m[5] * x0;
+      vr[j] = c0 - c1;
+      vi[j] = c2 + c3;
+      vr[k] = b0 - b1;
+      vi[k] = b2 + b3;
     }
-    vr[j] = o0.x;
-    vi[j] = o0.y;
-    vr[k] = o1.x;
-    vi[k] = o1.y;
   }
+}
+
+// Exchange two register values in place: each keeps its register, so a
+// permutation gate (CNOT, SWAP) leaves every group amplitude in the register
+// it had, and the gate variants of the dispatch switch agree on where each
+// amplitude lives (otherwise the join after the switch needs register
+// copies on every gate).  v_swap_b32 is an ALU exchange of two VGPRs.
+__device__ __forceinline__ void swap_in_place(double &a, double &b) {
+  const uint64_t ua = __builtin_bit_cast(uint64_t, a), ub = __builtin_bit_cast(uint64_t, b);
+  uint32_t alo = uint32_t(ua), ahi = uint32_t(ua >> 32), blo = uint32_t(ub), bhi = uint32_t(ub >> 32);
+  asm volatile("v_swap_b32 %0, %1" : "+v"(alo), "+v"(blo));
+  asm volatile("v_swap_b32 %0, %1" : "+v"(ahi), "+v"(bhi));
+  a = __builtin_bit_cast(double, (uint64_t(ahi) << 32) | alo);
+  b = __builtin_bit_cast(double, (uint64_t(bhi) << 32) | blo);
 }
 
 // Controlled gates: pairs (j, j | T) with bit C of j set.  SWAP: pairs
@@ -312,32 +343,27 @@ __device__ __forceinline__ void r_cu(double (&vr)[kGA], double (&vi)[kGA], const
     if (!(j & (1 << C)) || (j & (1 << T))) continue;
     if constexpr (CLS == kOpSwap) {
       const int k = (j ^ (1 << C)) | (1 << T);
-      const double ar = vr[j], ai = vi[j];
-      vr[j] = vr[k];
-      vi[j] = vi[k];
-      vr[k] = ar;
-      vi[k] = ai;
+      swap_in_place(vr[j], vr[k]);
+      swap_in_place(vi[j], vi[k]);
     } else {
       const int k = j | (1 << T);
-      const double2 v0 = make_double2(vr[j], vi[j]), v1 = make_double2(vr[k], vi[k]);
+      const double x0 = vr[j], y0 = vi[j], x1 = vr[k], y1 = vi[k];
       if constexpr (CLS == kOpCuDense) {
-        double2 o0, o1;
-        row2(m[0], m[1], m[2], m[3], v0.x, v0.y, v1.x, v1.y, o0.x, o0.y);
-        row2(m[4], m[5], m[6], m[7], v0.x, v0.y, v1.x, v1.y, o1.x, o1.y);
-        vr[j] = o0.x;
-        vi[j] = o0.y;
-        vr[k] = o1.x;
-        vi[k] = o1.y;
+        const double a0 = m[0] * x0, a1 = m[1] * y0, a2 = m[0] * y0, a3 = m[1] * x0;
+        const double b0 = m[4] * x0, b1 = m[5] * y0, b2 = m[4] * y0, b3 = m[5] * x0;
+        const double c0 = m[2] * x1, c1 = m[3] * y1, c2 = m[2] * y1, c3 = m[3] * x1;
+        const double d0 = m[6] * x1, d1 = m[7] * y1, d2 = m[6] * y1, d3 = m[7] * x1;
+        vr[j] = (a0 - a1) + (c0 - c1);
+        vi[j] = (a2 + a3) + (c2 + c3);
+        vr[k] = (b0 - b1) + (d0 - d1);
+        vi[k] = (b2 + b3) + (d2 + d3);
       } else if constexpr (CLS == kOpCnot) {
-        vr[j] = v1.x;
-        vi[j] = v1.y;
-        vr[k] = v0.x;
-        vi[k] = v0.y;
+        swap_in_place(vr[j], vr[k]);
+        swap_in_place(vi[j], vi[k]);
       } else {  // kOpCPhase
-        double2 o1;
-        cmul(m[6], m[7], v1.x, v1.y, o1.x, o1.y);
-        vr[k] = o1.x;
-        vi[k] = o1.y;
+        const double p0 = m[6] * x1, p1 = m[7] * y1, p2 = m[6] * y1, p3 = m[7] * x1;
+        vr[k] = p0 - p1;
+        vi[k] = p2 + p3;
       }
     }
   }
