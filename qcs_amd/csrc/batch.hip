@@ -65,7 +65,12 @@ enum : int {
   // kOp1Dense's (they may differ in the sign of a zero)
   kOp1Diag = 6,   // m01 = m10 = 0 (RZ)
   kOp1Real = 7,   // every imaginary part 0 (H, RY)
-  kOp1Anti = 8    // m00 = m11 = 0 (X, Y)
+  kOp1Anti = 8,   // m00 = m11 = 0 (X, Y)
+  // three gates on one position: diag(a, conj a), then the real rotation
+  // [[c, -s], [s, c]], then diag(b, conj b) (RZ, RY, RZ), in the op's m as
+  // {Re a, Im a, c, -s, Re b, Im b}; applied one after the other with the
+  // arithmetic of kOp1Diag, kOp1Real, kOp1Diag
+  kOp1Euler = 9
 };
 
 // Register groups (complex128): consecutive gates of a pass acting on at
@@ -309,6 +314,34 @@ __device__ __forceinline__ void r_1q(double (&vr)[kGA], double (&vi)[kGA], const
       vi[j] = a1 + c1;
       vr[k] = b0 + d0;
       vi[k] = b1 + d1;
+    } else if constexpr (CLS == kOp1Euler) {
+      // the entries the three matrices repeat are negations and copies of
+      // the ones held: -m7 = m1 and m6 = m0 (diagonal), m4 = -m2 and m6 = m0
+      // (rotation), so every product below is the one the separate ops form
+      double u0 = x0, v0 = y0, u1 = x1, v1 = y1;
+#pragma unroll
+      for (int d = 0; d < 2; ++d) {
+        const double ar = m[4 * d], ai = m[4 * d + 1], nai = -ai;
+        if (d == 1) {
+          const double c = m[2], s = m[3], ns = -s;
+          const double a0 = c * u0, a1 = c * v0, b0 = ns * u0, b1 = ns * v0;
+          const double c0 = s * u1, c1 = s * v1, d0 = c * u1, d1 = c * v1;
+          u0 = a0 + c0;
+          v0 = a1 + c1;
+          u1 = b0 + d0;
+          v1 = b1 + d1;
+        }
+        const double a0 = ar * u0, a1 = ai * v0, a2 = ar * v0, a3 = ai * u0;
+        const double d0 = ar * u1, d1 = nai * v1, d2 = ar * v1, d3 = nai * u1;
+        u0 = a0 - a1;
+        v0 = a2 + a3;
+        u1 = d0 - d1;
+        v1 = d2 + d3;
+      }
+      vr[j] = u0;
+      vi[j] = v0;
+      vr[k] = u1;
+      vi[k] = v1;
     } else {  // kOp1Anti
       const double c0 = m[2] * x1, c1 = m[3] * y1, c2 = m[2] * y1, c3 = m[3] * x1;
       const double b0 = m[4] * x0, b1 = m[5] * y0, b2 = m[4] * y0, b3 = m[5] * x0;
@@ -409,6 +442,7 @@ __device__ __forceinline__ void r_gate(double (&vr)[kGA], double (&vi)[kGA], con
     case kOp1Diag: r_1q_at<kOp1Diag>(vr, vi, t, m); break;
     case kOp1Real: r_1q_at<kOp1Real>(vr, vi, t, m); break;
     case kOp1Anti: r_1q_at<kOp1Anti>(vr, vi, t, m); break;
+    case kOp1Euler: r_1q_at<kOp1Euler>(vr, vi, t, m); break;
     case kOpCuDense: r_cu_at<kOpCuDense>(vr, vi, c, t, m); break;
     case kOpCnot: r_cu_at<kOpCnot>(vr, vi, c, t, m); break;
     case kOpCPhase: r_cu_at<kOpCPhase>(vr, vi, c, t, m); break;
@@ -1188,6 +1222,45 @@ static BGate make_bgate(const qcs_hip_pass &ps, const qcs_hip_gate &g) {
   return o;
 }
 
+static bool same_bits(double a, double b) {
+  uint64_t x, y;
+  std::memcpy(&x, &a, sizeof(x));
+  std::memcpy(&y, &b, sizeof(y));
+  return x == y;
+}
+// diag(a, conj a) and [[c, -s], [s, c]], entry for entry (bit patterns): the
+// repeated entries of an RZ and an RY
+static bool z_like(const BGate &o) {
+  return o.cls == kOp1Diag && same_bits(o.m[6], o.m[0]) && same_bits(o.m[7], -o.m[1]);
+}
+static bool y_like(const BGate &o) {
+  return o.cls == kOp1Real && same_bits(o.m[6], o.m[0]) && same_bits(o.m[4], -o.m[2]);
+}
+
+// One group's ops, with every run z-like, y-like, z-like on one position
+// made one kOp1Euler op (one dispatch on the device instead of three).
+static void emit_group_ops(const std::vector<BGate> &ops, std::vector<BGate> *bg) {
+  for (size_t x = 0; x < ops.size();) {
+    if (x + 2 < ops.size() && z_like(ops[x]) && y_like(ops[x + 1]) && z_like(ops[x + 2]) &&
+        ops[x + 1].t == ops[x].t && ops[x + 2].t == ops[x].t) {
+      BGate e;
+      std::memset(&e, 0, sizeof(e));
+      e.cls = kOp1Euler;
+      e.t = ops[x].t;
+      e.m[0] = ops[x].m[0];
+      e.m[1] = ops[x].m[1];
+      e.m[2] = ops[x + 1].m[0];
+      e.m[3] = ops[x + 1].m[2];
+      e.m[4] = ops[x + 2].m[0];
+      e.m[5] = ops[x + 2].m[1];
+      bg->push_back(e);
+      x += 3;
+    } else {
+      bg->push_back(ops[x++]);
+    }
+  }
+}
+
 // Register groups of one complex128 pass (tile_bits >= kGR): maximal runs of
 // consecutive gates on at most kGR tile positions, padded with the lowest
 // unused positions.
@@ -1221,8 +1294,8 @@ static void make_groups(const qcs_hip_pass &ps, const qcs_hip_gate *gates,
         if ((j >> i) & 1) off |= 1 << grp.qpos[i];
       grp.rowslot[j] = dswz(off);
     }
-    grp.first_gate = int(bg->size());
-    grp.ngates = g - start;
+    std::vector<BGate> ops;
+    ops.reserve(size_t(g - start));
     for (int x = start; x < g; ++x) {
       BGate o;
       std::memset(&o, 0, sizeof(o));
@@ -1230,8 +1303,11 @@ static void make_groups(const qcs_hip_pass &ps, const qcs_hip_gate *gates,
       o.c = gates[x].kind == QCS_HIP_G_1Q ? 0 : local[pos[gates[x].control]];
       std::memcpy(o.m, gates[x].m, sizeof(o.m));
       o.cls = gate_class(gates[x], true);
-      bg->push_back(o);
+      ops.push_back(o);
     }
+    grp.first_gate = int(bg->size());
+    emit_group_ops(ops, bg);
+    grp.ngates = int(bg->size()) - grp.first_gate;
     groups->push_back(grp);
   }
 }
