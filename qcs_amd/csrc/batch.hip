@@ -123,6 +123,9 @@ struct BGroup {
   int ngates;
   int qpos[kGR];       // tile positions of the group, ascending
   int rowslot[kGA];    // LDS slot (dswz) of the row pattern j
+  int outslot[kGA];    // slot register j is stored to: rowslot[j], or a
+                       // permutation of it that applies the group's
+                       // trailing CNOTs and SWAPs
 };
 
 struct BBlock {
@@ -473,7 +476,7 @@ __device__ void tile_group(double2 *L, int tb, const BGroup *__restrict__ gp,
     }
     for (int g = 0; g < ng; ++g) r_gate(vr, vi, g0 + g);
 #pragma unroll
-    for (int j = 0; j < kGA; ++j) L[base ^ rs[j]] = make_double2(vr[j], vi[j]);
+    for (int j = 0; j < kGA; ++j) L[base ^ gp->outslot[j]] = make_double2(vr[j], vi[j]);
   }
 }
 
@@ -1237,6 +1240,49 @@ static bool y_like(const BGate &o) {
   return o.cls == kOp1Real && same_bits(o.m[6], o.m[0]) && same_bits(o.m[4], -o.m[2]);
 }
 
+static bool is_perm(const BGate &o) { return o.cls == kOpCnot || o.cls == kOpSwap; }
+static unsigned op_positions(const BGate &o) {
+  return (1u << o.t) | (o.cls >= kOpCuDense && o.cls <= kOpSwap ? 1u << o.c : 0u);
+}
+
+// The register index map of a permutation op: after it, register x holds
+// what register perm_src(o, x) held.
+static int perm_src(const BGate &o, int x) {
+  const int bc = (x >> o.c) & 1, bt = (x >> o.t) & 1;
+  if (o.cls == kOpCnot) return bc ? x ^ (1 << o.t) : x;
+  // SWAP: exchange bits c and t
+  return bc == bt ? x : x ^ (1 << o.c) ^ (1 << o.t);
+}
+
+// CNOTs and SWAPs move amplitudes without arithmetic, so they commute
+// exactly with ops on other positions: each is moved past the later ops it
+// shares no position with, and the run that ends the group is applied by
+// storing the registers to permuted slots, outslot[F(j)] = rowslot[j] with
+// F(j) the register whose value ends at row j.  Returns the ops left.
+static std::vector<BGate> fold_trailing_perms(std::vector<BGate> ops, const int *rowslot,
+                                              int *outslot) {
+  const size_t n = ops.size();
+  for (bool moved = true; moved;) {
+    moved = false;
+    for (size_t i = n; i-- > 1;) {
+      BGate &a = ops[i - 1], &b = ops[i];
+      if (is_perm(a) && !is_perm(b) && !(op_positions(a) & op_positions(b))) {
+        std::swap(a, b);
+        moved = true;
+      }
+    }
+  }
+  size_t k = n;
+  while (k > 0 && is_perm(ops[k - 1])) --k;
+  for (int j = 0; j < kGA; ++j) {
+    int x = j;
+    for (size_t i = n; i-- > k;) x = perm_src(ops[i], x);
+    outslot[x] = rowslot[j];
+  }
+  ops.resize(k);
+  return ops;
+}
+
 // One group's ops, with every run z-like, y-like, z-like on one position
 // made one kOp1Euler op (one dispatch on the device instead of three).
 static void emit_group_ops(const std::vector<BGate> &ops, std::vector<BGate> *bg) {
@@ -1306,7 +1352,7 @@ static void make_groups(const qcs_hip_pass &ps, const qcs_hip_gate *gates,
       ops.push_back(o);
     }
     grp.first_gate = int(bg->size());
-    emit_group_ops(ops, bg);
+    emit_group_ops(fold_trailing_perms(std::move(ops), grp.rowslot, grp.outslot), bg);
     grp.ngates = int(bg->size()) - grp.first_gate;
     groups->push_back(grp);
   }

@@ -286,3 +286,40 @@ def test_shared_large_circuits_equal_standalone(qlib, cl, monkeypatch):
         assert np.array_equal(a2[i], own) and np.array_equal(i2[i], own_idx), n
         assert np.array_equal(a1[i], own) and np.array_equal(i1[i], own_idx), n
     b.close()
+
+
+def euler_and_permutation_ops(n, seed):
+    """Rotation triples RZ RY RZ on one qubit (one op on the device), CNOT
+    ladders and SWAPs that end register groups or sit inside them (moved
+    past the ops they share no qubit with, or stored through permuted
+    slots), and triples broken by a gate on their qubit."""
+    rng = np.random.default_rng(seed)
+    ops = C.Ops()
+    for layer in range(4):
+        for q in range(n):
+            ops.add(C.RZ, q, p=float(rng.uniform(-3, 3)))
+            ops.add(C.RY, q, p=float(rng.uniform(-3, 3)))
+            if (q + layer) % 5 == 0:
+                ops.add(C.CNOT, q, (q + 1) % n)  # inside the triple: no fusion
+            ops.add(C.RZ, q, p=float(rng.uniform(-3, 3)))
+        for q in range(n - 1):
+            ops.add(C.CNOT, q + 1, q)
+            if (q + layer) % 3 == 0:
+                ops.add(C.SWAP, q, (q + 2) % n)
+        ops.add(C.H, layer % n)
+    return ops
+
+
+@pytest.mark.parametrize("n", [5, 13, 14, 16])
+def test_c128_fused_rotations_and_folded_permutations(qlib, cl, n):
+    specs = [(n, euler_and_permutation_ops(n, 40 + n))]
+    b, amps, idx = batch_run(specs, qcs_amd.PLAIN_FP64, 256, cl, 11)
+    cl.srand(11)
+    c = qcs_amd.Circuit(n)
+    c.apply(specs[0][1])
+    ref_i = c.sample_indices(256)
+    ref_a = c.amplitudes()
+    c.close()
+    assert np.array_equal(amps[0], ref_a)
+    assert np.array_equal(idx[0], ref_i)
+    b.close()
