@@ -15,7 +15,7 @@ from ._native import (BATCH_GRAPH, BATCH_KEEP_STATES, BATCH_REPLAN, C64, C128, D
                       K_COMPOSE, K_COUNT, K_CU, K_FUSED, K_OTHER, K_REDUCE,
                       PLAIN_FP32, PLAIN_FP64, REFERENCE_COMPAT,
                       RENORM_AT_BARRIER, TILE_MATH_FP32, TILE_MATH_SPLIT_BF16,
-                      TUNE_TILE_GRID, TUNE_TILE_K4_FORM, TUNE_TILE_MATH, TUNE_TILE_THREADS,
+                      TUNE_TILE_DIRECT, TUNE_TILE_GRID, TUNE_TILE_K4_FORM, TUNE_TILE_MATH, TUNE_TILE_THREADS,
                       TILE_K4_CHAIN, TILE_K4_GAUSS, TILE_K4_HALVES,
                       HipBatchCircuit, HipBlock, HipGate, HipPass, libc, load)
 
@@ -25,7 +25,7 @@ __all__ = ["Circuit", "Batch", "load", "libc", "circuits", "PLAIN_FP64", "PLAIN_
            "FUSED_FP32", "REFERENCE_COMPAT", "RENORM_AT_BARRIER", "FUSE_K",
            "FUSE_TILES", "FUSE_SPLIT_BF16", "HipPass", "TUNE_TILE_THREADS", "TUNE_TILE_GRID",
            "TUNE_TILE_MATH", "TILE_MATH_FP32", "TILE_MATH_SPLIT_BF16",
-           "TUNE_TILE_K4_FORM", "TILE_K4_CHAIN", "TILE_K4_HALVES", "TILE_K4_GAUSS",
+           "TUNE_TILE_K4_FORM", "TUNE_TILE_DIRECT", "TILE_K4_CHAIN", "TILE_K4_HALVES", "TILE_K4_GAUSS",
            "C128", "C64", "HIP_COMPAT", "HipGate", "HipBlock", "K_1Q", "K_CU",
            "K_FUSED", "K_COMPOSE", "K_REDUCE", "K_OTHER", "K_COUNT"]
 

@@ -52,6 +52,7 @@ class HipPass(ctypes.Structure):
 
 TILE_BITS, TILE_LOW_BITS = 13, 4
 TUNE_TILE_THREADS, TUNE_TILE_GRID, TUNE_TILE_MATH, TUNE_TILE_K4_FORM = 0, 1, 2, 3
+TUNE_TILE_DIRECT = 4
 TILE_K4_CHAIN, TILE_K4_HALVES, TILE_K4_GAUSS = 0, 1, 2
 TILE_MATH_FP32, TILE_MATH_SPLIT_BF16 = 0, 1
 FUSE_TILES = 0x10

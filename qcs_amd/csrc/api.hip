@@ -651,6 +651,11 @@ int qcs_hip_set_tuning(qcs_hip_state *s, int knob, int value) {
     s->tile_k4 = value;
     return 0;
   }
+  if (knob == QCS_HIP_TUNE_TILE_DIRECT) {
+    if (value != 0 && value != 1) return fail(QCS_HIP_EINVAL, "tile direct must be 0 or 1");
+    s->tile_direct = value;
+    return 0;
+  }
   if (knob == QCS_HIP_TUNE_TILE_GRID) {
     if (value < 8 || value > 65536 || (value & 7))
       return fail(QCS_HIP_EINVAL, "tile grid must be a multiple of 8 in [8, 65536]");
@@ -668,6 +673,7 @@ int qcs_hip_get_tuning(qcs_hip_state *s, int knob, int *value) {
   else if (knob == QCS_HIP_TUNE_TILE_GRID) *value = s->tile_grid;
   else if (knob == QCS_HIP_TUNE_TILE_MATH) *value = s->tile_math;
   else if (knob == QCS_HIP_TUNE_TILE_K4_FORM) *value = s->tile_k4;
+  else if (knob == QCS_HIP_TUNE_TILE_DIRECT) *value = s->tile_direct;
   else if (knob >= QCS_HIP_TUNE_BLOCK_VARIANT(1) &&
            knob <= QCS_HIP_TUNE_BLOCK_VARIANT(QCS_HIP_MAX_FUSE))
     *value = s->block_variant[knob - QCS_HIP_TUNE_BLOCK_VARIANT(0)];

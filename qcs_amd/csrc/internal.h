@@ -75,6 +75,8 @@ struct qcs_hip_state {
   int tile_grid = 256;      // persistent grid of the tile-program kernel
   int tile_math = QCS_HIP_TILE_MATH_FP32;  // block arithmetic of tile programs
   int tile_k4 = QCS_HIP_TILE_K4_GAUSS;     // form of k = 4 blocks in tile programs
+  int tile_direct = 1;  // three-product tile programs: first block from HBM,
+                        // last block to HBM, where the pass allows
   // kernel timing
   int timing = 0;
   std::vector<qcs::TimedLaunch> pending_events;

@@ -105,7 +105,8 @@ struct TilePass {
   int tb;
   int nblocks;
   int first_block;
-  int reserved;
+  int direct;  // 1: the first block's operands come from HBM and the last
+               // block's results go to HBM (the three-product kernel)
   int sbits[kPassMaxBits];
 };
 
@@ -128,6 +129,12 @@ struct TileBlock {
   int wpart[16];             // 4 swz(wave index on the wave bits)
   int jpart[16];             // 4 swz(j on the group bits)
   int lanecol[32];           // 4 swz(l32 on the lane bits)
+  // the same tables as tile element indices (no swizzle, not in bytes), for
+  // the first and last blocks of a direct pass, whose operands and results
+  // move between registers and HBM
+  int elane[32];             // l32 on the lane bits
+  int ewave[16];             // wave index on the wave bits
+  int erow[kTileKMax];       // target q
 };
 
 __host__ __device__ __forceinline__ int swz(int e) {
@@ -143,6 +150,7 @@ __device__ __forceinline__ void st_nt(float2 *p, float4 x) {
   const f4v v = {x.x, x.y, x.z, x.w};
   __builtin_nontemporal_store(v, reinterpret_cast<f4v *>(p));
 }
+
 
 // LDS by absolute byte address (the dynamic LDS starts at 0)
 template <typename T>
@@ -644,6 +652,37 @@ __device__ __forceinline__ void tp_block4g(const BlockScalars &d, const BlockReg
   }
 }
 
+// The last block of a direct pass: the three-product block with its results
+// written to HBM (rows q | v << 2 of the lane's column, both halves in one
+// 16-byte store, at `base` | the lane's offset o | the row offsets of target
+// bits 2 and 3), the imaginary parts negated back; the next tile's first
+// block's scalars, A operands and lane addresses are read as usual.
+template <int NTH, int TILE>
+__device__ __forceinline__ void tp_block4g_out(const BlockScalars &d, const BlockRegs &r,
+                                               float sgn, const TileBlock *__restrict__ nbl,
+                                               int nblk, int wave, BlockScalars &nd,
+                                               BlockRegs &nr, const K4GState &st, float2 *a,
+                                               uint64_t base, uint64_t o, uint64_t m2,
+                                               uint64_t m3) {
+  static_assert((TILE >> 4) / 32 / (NTH / 64) == 1, "one 32-column group per wave");
+  const i32x2 nl = load_lanes4g(nblk);
+  __builtin_amdgcn_sched_barrier(0);
+  f4v re0, nim0, re1, nim1;
+  k4g_chain(r, st.b0, sgn, re0, nim0);
+  __builtin_amdgcn_sched_barrier(0);
+  nd = load_scalars(nbl, wave);
+  __builtin_amdgcn_sched_barrier(0);
+  k4g_chain(r, st.b1, sgn, re1, nim1);
+  __builtin_amdgcn_sched_barrier(0);
+  load_a4g(nr, nblk);
+  nr.lp = nl[0];
+  nr.lp2 = nl[1];
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+    st_nt(a + (base | o | ((v & 1) ? m2 : 0) | ((v & 2) ? m3 : 0)),
+          make_float4(re0[v], -nim0[v], re1[v], -nim1[v]));
+}
+
 // One FP32 block of the pass (index b), with the next block's values read
 // into (nd, nr); the block after the last is the pass's first (next tile).
 template <int NTH, int TILE>
@@ -739,13 +778,18 @@ __device__ __forceinline__ void tp_block_split(const TileBlock *__restrict__ bl,
 // kernel); vector loads of them would wait behind the tile prefetch.
 // K4: how FP32 k = 4 blocks run -- 0: the 32x32 chain (k <= 3 blocks on
 // the 16-column form), 1: pipelined halves, 2: pipelined three-product
-// halves (1 and 2 run every block as a k = 4 block)
+// halves (1 and 2 run every block as a k = 4 block), 3: 2 on a direct pass
+// (the first block's operands loaded from HBM into registers, the last
+// block's results stored from registers to HBM: no tile fill or read-out
+// through LDS)
 template <int TB, int NTH, bool SPLIT, int K4>
 __global__ __launch_bounds__(NTH) void k_tile_program(
     float2 *__restrict__ a, const TilePass *__restrict__ passes, int pass_index,
     const TileBlock *__restrict__ blocks, const float *__restrict__ wbuf,
     uint64_t ntiles) {
   constexpr int TILE = 1 << TB;
+  constexpr bool DIR = K4 == 3;
+  constexpr int KF = DIR ? 2 : K4;  // the block form
   constexpr int TBITS = __builtin_ctz(NTH) + 1;  // tile bits covered by 2*tid
   constexpr int IT = TILE / (2 * NTH);           // float4 loads per thread
   constexpr int IB = TB - TBITS;                 // tile bits covered by `it`
@@ -769,7 +813,7 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
     const int R = 1 << bl.k, RR = 2 * R, row = ln & 31, hh = ln >> 5;
     const float *W = wbuf + size_t(bl.wslot) * kTileWFloats;
     float w = (row < RR && s < R) ? W[row * RR + 2 * s + hh] : 0.0f;
-    if (!SPLIT && K4 == 2) {
+    if (!SPLIT && KF == 2) {
       // three-product form: a[p][t] = A[i][qq] of product p (Ur + Ui,
       // -s (Ur - Ui) with s = +1 for the pass's first block and -1 after,
       // -Ui), instruction t; D row i = 4 qi + vi means output row qi | vi << 2
@@ -788,7 +832,7 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
         const int lrows = ((ln & 16) ? bl.rowoff[1] : 0) ^ ((ln & 32) ? bl.rowoff[2] : 0);
         w = t < 2 ? __int_as_float(bl.lanecol[(ln & 15) | (t << 4)] ^ lrows) : 0.0f;
       }
-    } else if (!SPLIT && (bl.k <= 3 || K4 == 1)) {
+    } else if (!SPLIT && (bl.k <= 3 || KF == 1)) {
       // the 16-column form (tp_block16, tp_block4p): instruction sp = s & 7
       // of row block rb = s >> 3, A[i][qq] with i = 4 qi + vi holding output
       // row (qi & 1) | (vi >> 1) << 1 | (qi >> 1) << 2 | rb << 3, part vi & 1,
@@ -859,6 +903,84 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
   const uint64_t cmask = ((ntiles << TB) - 1) & ~tmask;
   const uint64_t dinc = deposit(G8);
 
+  if constexpr (DIR) {
+    static_assert(!SPLIT && NTH == 1024, "direct passes: the three-product form");
+    // lane (q, c) of wave w holds, in column half h, rows q | t << 2 (t =
+    // 0..3) of one column of the first and of the last block: tile element
+    // = the lane part (c on tile positions 1..4, h on position 0) ^ the rows
+    // of target bits 0, 1 (q) ^ the wave part ^ the rows of target bits 2, 3
+    // (t); as index bits, the element's bits deposited on the pass's tile
+    // bits
+    auto dep = [&](int e) {
+      uint64_t x = 0;
+#pragma unroll
+      for (int j = 0; j < TB; ++j)
+        if ((e >> j) & 1) x |= uint64_t(1) << ps.sbits[j];
+      return x;
+    };
+    const TileBlock &B0 = blocks[ps.first_block], &BL = blocks[ps.first_block + nb - 1];
+    const int lane = tid & 63, c16 = lane & 15, q = lane >> 4;
+    const int fq = ((q & 1) ? B0.erow[0] : 0) ^ ((q & 2) ? B0.erow[1] : 0) ^ B0.ewave[wave];
+    const int lq = ((q & 1) ? BL.erow[0] : 0) ^ ((q & 2) ? BL.erow[1] : 0) ^ BL.ewave[wave];
+    // (half 1 is half 0 + 1: the half bit is tile position 0, index bit 0)
+    const uint64_t fo = dep(B0.elane[c16] ^ fq), lo = dep(BL.elane[c16] ^ lq);
+    const uint64_t fr2 = dep(B0.erow[2]), fr3 = dep(B0.erow[3]);
+    const uint64_t lr2 = dep(BL.erow[2]), lr3 = dep(BL.erow[3]);
+    // the first block's operands of the tile at b: row t -> (re, im) of
+    // half 0, (re, im) of half 1
+    auto load_ops = [&](float4 (&x)[4], uint64_t b) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+        x[t] = ld_nt(a + (b | fo | ((t & 1) ? fr2 : 0) | ((t & 2) ? fr3 : 0)));
+    };
+    auto step_base = [&](uint64_t b) { return ((b | ~cmask) + dinc) & cmask; };
+    uint64_t base = deposit(tile);
+    uint64_t nbase = step_base(base), nbase2 = step_base(nbase);
+    // two tiles in flight, in two register sets taken in turn; past the
+    // end a load reads the current tile again, unused
+    float4 xa[4], xb[4];
+    if (tile < tend) {
+      load_ops(xa, base);
+      load_ops(xb, tile + G8 < tend ? nbase : base);
+    }
+    __syncthreads();  // operands staged
+    BlockScalars sa = load_scalars(&blocks[ps.first_block], wave), sb;
+    BlockRegs ra;
+    load_regs4g(ra, OPS);
+    const TileBlock *fb = &blocks[ps.first_block];
+    auto step = [&](float4 (&cur)[4]) {
+      K4GState st;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        st.b0[t] = cur[t].x;
+        st.b0[4 + t] = cur[t].y;
+        st.b1[t] = cur[t].z;
+        st.b1[4 + t] = cur[t].w;
+      }
+      load_ops(cur, tile + 2 * G8 < tend ? nbase2 : base);
+      for (int b = 0; b + 1 < nb; ++b) {
+        tp_block4g<NTH, TILE>(sa, ra, b == 0 ? 1.0f : -1.0f, fb + b + 1,
+                              OPS + (b + 1) * kTileStrideFp32, wave, sb, ra, st);
+        sa = sb;
+      }
+      tp_block4g_out<NTH, TILE>(sa, ra, nb == 1 ? 1.0f : -1.0f, fb, OPS, wave, sb, ra, st, a,
+                                base, lo, lr2, lr3);
+      sa = sb;
+      // the next tile's first block writes the LDS tile its last block read
+      __syncthreads();
+      tile += G8;
+      base = nbase;
+      nbase = nbase2;
+      nbase2 = step_base(nbase2);
+    };
+    while (tile < tend) {
+      step(xa);
+      if (tile >= tend) break;
+      step(xb);
+    }
+    return;
+  }
+
   // The next tile is loaded into registers while the current one is
   // computed.  At a tile boundary the finished tile is read out of LDS, the
   // prefetched one written in, and only then are the stores issued: stores
@@ -868,7 +990,7 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
   // DEPTH tiles are in flight: the three-product form loads tile t + 2
   // while it computes tile t (two register sets taken in turn), so a pass
   // whose tiles are scattered over many pages still streams at the HBM rate.
-  constexpr int DEPTH = (!SPLIT && K4 == 2) ? 2 : 1;
+  constexpr int DEPTH = (!SPLIT && KF == 2) ? 2 : 1;
   float4 v0[IT], v1[IT];
   uint64_t base = deposit(tile);
   auto step_base = [&](uint64_t b) { return ((b | ~cmask) + dinc) & cmask; };
@@ -882,9 +1004,9 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
   BlockRegs ra, rb;
   if (!SPLIT) {
     sa = load_scalars(&blocks[ps.first_block], wave);
-    if (K4 == 2)
+    if (KF == 2)
       load_regs4g(ra, OPS);
-    else if (K4 == 1)
+    else if (KF == 1)
       load_regs4p(ra, OPS);
     else
       load_regs(ra, OPS);
@@ -943,11 +1065,11 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
       // two register sets taken in turn, so the values read one block ahead
       // need no copies; (sa, ra) hold the pass's first block on entry
       const TileBlock *fb = &blocks[ps.first_block];
-      if constexpr (K4 != 0) {
+      if constexpr (KF != 0) {
         // every block is a k = 4 block (the host pads smaller ones); one set
         // of per-lane values, read in place once a block's last chain is
         // issued; the scalars alternate between two sets
-        if constexpr (K4 == 2) {
+        if constexpr (KF == 2) {
           K4GState st;
           read_k4g_both(sa, ra, st);
           for (int b = 0; b < nb; ++b) {
@@ -985,7 +1107,7 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
       const int s0 = slot_t ^ swz(it << TBITS);
-      if constexpr (K4 == 2)  // the three-product form leaves -Im in the tile
+      if constexpr (KF == 2)  // the three-product form leaves -Im in the tile
         o[it] = make_float4(re[s0], -im[s0], re[s0 ^ 1], -im[s0 ^ 1]);
       else
         o[it] = make_float4(re[s0], im[s0], re[s0 ^ 1], im[s0 ^ 1]);
@@ -1147,6 +1269,14 @@ static int target_mask(const qcs_hip_pass &ps, const qcs_hip_block &blk) {
   return m;
 }
 
+// value whose bit i sits at position pos[i], as a tile element index
+static int scatter_elem(int v, const int *pos, int npos) {
+  int e = 0;
+  for (int i = 0; i < npos; ++i)
+    if ((v >> i) & 1) e |= 1 << pos[i];
+  return e;
+}
+
 // value whose bit i sits at position pos[i], through swz, as a byte offset
 static int scatter_off(int v, const int *pos, int npos) {
   int e = 0;
@@ -1161,7 +1291,7 @@ static int scatter_off(int v, const int *pos, int npos) {
 // else the lowest ones; form16: the block runs in the 16-column form.
 static void make_tile_block(const qcs_hip_pass &ps, const qcs_hip_block &blk, int wslot,
                             int wmask, int nwb, bool sync, bool form16, int half,
-                            TileBlock *tb) {
+                            TileBlock *tb, bool direct = false) {
   std::memset(tb, 0, sizeof(TileBlock));
   const int k = blk.k, R = 1 << k;
   tb->k = k;
@@ -1175,10 +1305,17 @@ static void make_tile_block(const qcs_hip_pass &ps, const qcs_hip_block &blk, in
   for (int p = 0; p < kTileBitsFixed; ++p)
     if ((wmask >> p) & 1) wpos[nw++] = p;
   int avail = ((1 << kTileBitsFixed) - 1) & ~tmask & ~wmask;
-  if (form16)
+  if (direct) {
+    // the half bit on tile position 0 and lane bits 0..3 on positions 1..4
+    // (the lowest index bits): a lane's two halves of a row are one 16-byte
+    // access, and 16 lanes cover 256 contiguous bytes of HBM
+    for (int i = 0; i < 4; ++i) lanes[i] = i + 1;
+    lanes[4] = 0;
+  } else if (form16) {
     (void)pick_lanes16(avail, bank_bit(tpos[0]), lanes, half);
-  else
+  } else {
     (void)pick_lanes(avail, lanes);
+  }
   for (int i = 0; i < 5; ++i) avail &= ~(1 << lanes[i]);
   int nj = 0;
   for (int p = 0; p < kTileBitsFixed; ++p)
@@ -1186,6 +1323,9 @@ static void make_tile_block(const qcs_hip_pass &ps, const qcs_hip_block &blk, in
   for (int r = 0; r < R; ++r) tb->rowoff[r] = scatter_off(r, tpos, k);
   for (int l = 0; l < 32; ++l) tb->lanecol[l] = scatter_off(l, lanes, 5);
   for (int w = 0; w < (1 << nwb); ++w) tb->wpart[w] = scatter_off(w, wpos, nwb);
+  for (int l = 0; l < 32; ++l) tb->elane[l] = scatter_elem(l, lanes, 5);
+  for (int w = 0; w < (1 << nwb); ++w) tb->ewave[w] = scatter_elem(w, wpos, nwb);
+  for (int q = 0; q < k; ++q) tb->erow[q] = 1 << tpos[q];
   for (int j = 0; j < (1 << nj); ++j) tb->jpart[j] = scatter_off(j, jpos, nj);
 }
 
@@ -1195,8 +1335,8 @@ static void make_tile_block(const qcs_hip_pass &ps, const qcs_hip_block &blk, in
 // trade barriers against LDS bank conflicts, by dynamic programming over the
 // pass.  With FP32 math, blocks of k <= 3 run in the 16-column form, whose
 // conflicts also depend on the first target.
-static void layout_pass(const qcs_hip_pass &ps, const qcs_hip_block *blocks, int nwb,
-                        bool split, bool k4p, TileBlock *tb) {
+static bool layout_pass(const qcs_hip_pass &ps, const qcs_hip_block *blocks, int nwb,
+                        bool split, bool k4p, TileBlock *tb, bool direct = false) {
   // conflict factor of every set of free positions (and, for the 16-column
   // form, bank bit of the first target), and the candidate wave bits of each
   // width, computed once per process
@@ -1273,9 +1413,14 @@ static void layout_pass(const qcs_hip_pass &ps, const qcs_hip_block *blocks, int
             arg_of_mask[j] = j * nh + h;
           }
       }
+    // a direct pass's first and last blocks keep tile positions 0..4 for
+    // the half bit (position 0) and the lanes: no wave bit there
+    const bool edge = direct && (i == 0 || i + 1 == nblk);
     for (int j = 0; j < nm; ++j) {
       if (cand[j] & tm[i]) continue;
+      if (edge && (cand[j] & 31)) continue;
       for (int h = 0; h < nh; ++h) {
+        if (edge && h != 0) continue;
         const int cf = conflict(i, cand[j], h);
         if (cf < 0) continue;
         const int sidx = j * nh + h;
@@ -1296,6 +1441,7 @@ static void layout_pass(const qcs_hip_pass &ps, const qcs_hip_block *blocks, int
         }
       }
     }
+    if (arg < 0) return false;  // no layout under the direct constraints
     prev_best = best;
     prev_arg = arg;
   }
@@ -1309,7 +1455,8 @@ static void layout_pass(const qcs_hip_pass &ps, const qcs_hip_block *blocks, int
   for (int i = 0; i < nblk; ++i) {
     const int bi = ps.first_block + i;
     const bool last = i + 1 == nblk || wmask[i + 1] != wmask[i];
-    make_tile_block(ps, blocks[bi], bi, wmask[i], nwb, last, form16(bi), whalf[i], &tb[bi]);
+    make_tile_block(ps, blocks[bi], bi, wmask[i], nwb, last, form16(bi), whalf[i], &tb[bi],
+                    direct && (i == 0 || i + 1 == nblk));
   }
   // a k = 4 block reads the next block's first half ahead when that block is
   // also k = 4, in this run, with the same half bit
@@ -1320,6 +1467,7 @@ static void layout_pass(const qcs_hip_pass &ps, const qcs_hip_block *blocks, int
                        ? 1
                        : 0;
   }
+  return true;
 }
 
 // Blocks of fewer than 4 qubits widened to 4 (their unitary composes to
@@ -1367,6 +1515,10 @@ int tiles_apply(qcs_hip_state *s, const qcs_hip_pass *passes, int np,
   // neighbouring blocks' targets where possible so no run needs more bits
   const bool k4p = !split_math && s->tile_threads == 1024 &&
                    (s->tile_k4 == QCS_HIP_TILE_K4_HALVES || s->tile_k4 == QCS_HIP_TILE_K4_GAUSS);
+  // the three-product kernel's direct form (first block's operands from HBM,
+  // last block's results to HBM)
+  const bool direct_ok = k4p && s->tile_threads == 1024 &&
+                         s->tile_k4 == QCS_HIP_TILE_K4_GAUSS && s->tile_direct != 0;
   std::vector<qcs_hip_block> padded;
   const qcs_hip_block *lay = blocks;
   if (k4p) {
@@ -1384,7 +1536,15 @@ int tiles_apply(qcs_hip_state *s, const qcs_hip_pass *passes, int np,
     tp[p].nblocks = passes[p].num_blocks;
     tp[p].first_block = passes[p].first_block;
     for (int j = 0; j < passes[p].tile_bits; ++j) tp[p].sbits[j] = passes[p].bits[j];
-    layout_pass(passes[p], lay, nwb, split_math, k4p, tb);
+    // direct when the first and last blocks leave tile positions 0..4 (the
+    // lowest index bits, in every pass) to the half bit and the lanes
+    const int nbp = passes[p].num_blocks, fb = passes[p].first_block;
+    const bool dir = direct_ok && nbp > 0 && !(target_mask(passes[p], lay[fb]) & 31) &&
+                     !(target_mask(passes[p], lay[fb + nbp - 1]) & 31);
+    if (dir && layout_pass(passes[p], lay, nwb, split_math, k4p, tb, true))
+      tp[p].direct = 1;
+    else
+      layout_pass(passes[p], lay, nwb, split_math, k4p, tb);
   }
   QCS_HIP_TRY(hipMemcpyAsync(s->tpass_dev, tp, pbytes, hipMemcpyHostToDevice,
                              s->stream));
@@ -1410,7 +1570,8 @@ int tiles_apply(qcs_hip_state *s, const qcs_hip_pass *passes, int np,
       rc = split ? launch_tp<1024, true, 0>(s, dp, p, db, ntiles, nbp)
            : !k4p ? launch_tp<1024, false, 0>(s, dp, p, db, ntiles, nbp)
            : s->tile_k4 == QCS_HIP_TILE_K4_GAUSS
-               ? launch_tp<1024, false, 2>(s, dp, p, db, ntiles, nbp)
+               ? (tp[p].direct ? launch_tp<1024, false, 3>(s, dp, p, db, ntiles, nbp)
+                               : launch_tp<1024, false, 2>(s, dp, p, db, ntiles, nbp))
                : launch_tp<1024, false, 1>(s, dp, p, db, ntiles, nbp);
     else
       rc = split ? launch_tp<512, true, 0>(s, dp, p, db, ntiles, nbp)

@@ -225,6 +225,56 @@ def test_tile_k4_halves_form(qlib, n, seed):
         assert hip.qcs_hip_set_tuning(a.device_state, qcs_amd.TUNE_TILE_K4_FORM, 3) == -1
 
 
+def direct_passes(passes, blocks):
+    """Passes whose first and last blocks (before padding) leave qubits 0..4
+    to the lanes: the ones the direct form can take."""
+    def off_low(b):
+        return not set(range(5)) & set(b.targets[:b.k])
+    return [p for p in passes if p.num_blocks > 0 and off_low(blocks[p.first_block])
+            and off_low(blocks[p.first_block + p.num_blocks - 1])]
+
+
+@pytest.mark.parametrize("n,seed", [(16, 1), (19, 2), (22, 3)])
+def test_tile_direct_passes_bit_identical(qlib, n, seed):
+    """Direct passes of the three-product form (the first block's operands
+    loaded from HBM into registers, the last block's results stored from
+    registers, on passes whose first and last blocks leave qubits 0..4 to
+    the lanes) compute the same products as the LDS path: the states are
+    bit-identical with the direct form on and off.  An operand or result
+    at a wrong address, row or half would differ by O(1)."""
+    from test_fusion_plan import pass_plan
+    hip, _ = qlib
+    print(f"[fused] tile direct n={n}", flush=True)
+    kinds = [C.H, C.X, C.Y, C.Z, C.P, C.RX, C.RY, C.RZ, C.CNOT, C.CPHASE, C.SWAP]
+    for i, o in enumerate((C.layered(n, 6, seed),
+                           C.random_circuit(n, 400, 31 * seed + n, kinds=kinds))):
+        passes, blocks, ordered = pass_plan(o, n)
+        if i == 0:  # the layered circuit has passes the direct form takes
+            assert direct_passes(passes, blocks)
+        P = (qcs_amd.HipPass * len(passes))(*passes)
+        B = (qcs_amd.HipBlock * len(blocks))(*blocks)
+        G = (qcs_amd.HipGate * len(ordered))(*ordered)
+        psi = rand_state(n, np.random.default_rng(3 * n + seed))
+        got = []
+        for on in (1, 0):
+            with qcs_amd.Circuit(n, qcs_amd.PLAIN_FP32) as st:
+                st.set_amplitudes(psi)
+                assert hip.qcs_hip_set_tuning(st.device_state, qcs_amd.TUNE_TILE_K4_FORM,
+                                              qcs_amd.TILE_K4_GAUSS) == 0
+                assert hip.qcs_hip_set_tuning(st.device_state, qcs_amd.TUNE_TILE_DIRECT,
+                                              on) == 0
+                rc = hip.qcs_hip_apply_passes(st.device_state, P, len(passes), B,
+                                              len(blocks), G, len(ordered))
+                assert rc == 0, hip.qcs_hip_last_error()
+                got.append(st.amplitudes())
+        assert np.array_equal(got[0], got[1]), (n, i, len(direct_passes(passes, blocks)))
+    with qcs_amd.Circuit(n, qcs_amd.PLAIN_FP32) as a:
+        assert hip.qcs_hip_set_tuning(a.device_state, qcs_amd.TUNE_TILE_DIRECT, 2) == -1
+        v = ctypes.c_int()
+        assert hip.qcs_hip_get_tuning(a.device_state, qcs_amd.TUNE_TILE_DIRECT,
+                                      ctypes.byref(v)) == 0 and v.value == 1
+
+
 def test_apply_passes_rejects_bad_plans(qlib):
     from test_fusion_plan import pass_plan
     hip, _ = qlib
