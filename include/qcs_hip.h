@@ -261,7 +261,7 @@ int qcs_hip_apply_passes(qcs_hip_state *s, const qcs_hip_pass *passes,
  * form reads a pass's first block's operands straight from HBM into
  * registers and writes its last block's results straight back, wherever
  * both blocks leave the 5 lowest qubits to the lanes (no LDS round trip at
- * the tile boundaries); the results are the same either way. */
+ * the tile boundaries); the results agree to FP32 rounding either way. */
 #define QCS_HIP_TUNE_TILE_DIRECT 4
 /* QCS_HIP_TUNE_BLOCK_VARIANT(k): launch variant of the one-block-per-pass
  * matrix-core kernel for blocks of k qubits (1..5), one of: */

@@ -135,6 +135,10 @@ struct TileBlock {
   int elane[32];             // l32 on the lane bits
   int ewave[16];             // wave index on the wave bits
   int erow[kTileKMax];       // target q
+  // row bit q of the kernel's row patterns is bit tidx[q] of the composed
+  // unitary's (whose bit j is the j-th lowest target): the pipelined k = 4
+  // forms put the target that minimises bank conflicts on row bit 0
+  int tidx[kTileKMax];
 };
 
 __host__ __device__ __forceinline__ int swz(int e) {
@@ -773,6 +777,15 @@ __device__ __forceinline__ void tp_block_split(const TileBlock *__restrict__ bl,
   }
 }
 
+// a kernel row pattern as a row of the composed unitary (TileBlock::tidx)
+__device__ __forceinline__ int sigma4(const TileBlock &bl, int r) {
+  int x = 0;
+#pragma unroll
+  for (int q = 0; q < kTileKMax; ++q)
+    if ((r >> q) & 1) x |= 1 << bl.tidx[q];
+  return x;
+}
+
 // The descriptor pointers are __restrict__ so the compiler may read them
 // through the scalar cache (they are uniform and never written by the
 // kernel); vector loads of them would wait behind the tile prefetch.
@@ -821,7 +834,7 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
       // address in column half 0 and 1, lane table entry (ln & 15) | 16 h ^
       // the row offsets of target bits 0 and 1 on lane bits 4 and 5.
       const int i = ln & 15, qq = ln >> 4, p = s >> 2, t = s & 3;
-      const int ro = (i >> 2) | ((i & 3) << 2), si = qq | (t << 2);
+      const int ro = sigma4(bl, (i >> 2) | ((i & 3) << 2)), si = sigma4(bl, qq | (t << 2));
       if (p < 3) {
         const bool in = ro < R && si < R;
         w = !in ? 0.0f
@@ -838,8 +851,8 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
       // row (qi & 1) | (vi >> 1) << 1 | (qi >> 1) << 2 | rb << 3, part vi & 1,
       // and qq meaning input row 2 sp + (qq & 1), part qq >> 1
       const int i = ln & 15, qq = ln >> 4, qi = i >> 2, vi = i & 3, rb = s >> 3, sp = s & 7;
-      const int ro = (qi & 1) | ((vi >> 1) << 1) | ((qi >> 1) << 2) | (rb << 3);
-      const int si = 2 * sp + (qq & 1);
+      const int ro = sigma4(bl, (qi & 1) | ((vi >> 1) << 1) | ((qi >> 1) << 2) | (rb << 3));
+      const int si = sigma4(bl, 2 * sp + (qq & 1));
       w = (ro < R && si < R) ? W[(2 * ro + (vi & 1)) * RR + 2 * si + (qq >> 1)] : 0.0f;
     }
     const int blk = OPS + b * STRIDE;
@@ -1291,7 +1304,7 @@ static int scatter_off(int v, const int *pos, int npos) {
 // else the lowest ones; form16: the block runs in the 16-column form.
 static void make_tile_block(const qcs_hip_pass &ps, const qcs_hip_block &blk, int wslot,
                             int wmask, int nwb, bool sync, bool form16, int half,
-                            TileBlock *tb, bool direct = false) {
+                            TileBlock *tb, bool direct = false, bool choose_t0 = false) {
   std::memset(tb, 0, sizeof(TileBlock));
   const int k = blk.k, R = 1 << k;
   tb->k = k;
@@ -1305,10 +1318,37 @@ static void make_tile_block(const qcs_hip_pass &ps, const qcs_hip_block &blk, in
   for (int p = 0; p < kTileBitsFixed; ++p)
     if ((wmask >> p) & 1) wpos[nw++] = p;
   int avail = ((1 << kTileBitsFixed) - 1) & ~tmask & ~wmask;
+  // the target on row bit 0 (lane bit 4's rows): the one with the fewest
+  // bank conflicts of a half-wave's accesses; the others keep their order
+  int tidx[kTileKMax];
+  for (int q = 0; q < k; ++q) tidx[q] = q;
+  if (choose_t0) {
+    int best = 0, best_f = 1 << 30;
+    for (int q = 0; q < k; ++q) {
+      int f, tmp[5];
+      if (direct) {
+        f = bank_bit(tpos[q]) == 0 ? 1 : 2;  // lane bits on banks 1..4
+      } else {
+        f = pick_lanes16(avail, bank_bit(tpos[q]), tmp, half);
+      }
+      if (f < best_f) {
+        best_f = f;
+        best = q;
+      }
+    }
+    for (int q = 0, j = 1; q < k; ++q)
+      if (q != best) tidx[j++] = q;
+    tidx[0] = best;
+    int t2[kTileKMax];
+    for (int q = 0; q < k; ++q) t2[q] = tpos[tidx[q]];
+    for (int q = 0; q < k; ++q) tpos[q] = t2[q];
+  }
+  for (int q = 0; q < kTileKMax; ++q) tb->tidx[q] = q < k ? tidx[q] : q;
   if (direct) {
     // the half bit on tile position 0 and lane bits 0..3 on positions 1..4
     // (the lowest index bits): a lane's two halves of a row are one 16-byte
-    // access, and 16 lanes cover 256 contiguous bytes of HBM
+    // access, and 16 lanes cover 256 contiguous bytes of HBM (8 lanes, one
+    // 128-byte line, measured 2 % slower)
     for (int i = 0; i < 4; ++i) lanes[i] = i + 1;
     lanes[4] = 0;
   } else if (form16) {
@@ -1381,12 +1421,27 @@ static bool layout_pass(const qcs_hip_pass &ps, const qcs_hip_block *blocks, int
     const int t = tm[i];
     int a = (kSets - 1) & ~t & ~mk;
     int f;
+    if (direct && (i == 0 || i + 1 == nblk)) {
+      // the direct lanes of make_tile_block (banks 1..4): conflict-free when
+      // a target on bank 0 can take row bit 0
+      f = 2;
+      for (int p = 0; p < kTileBitsFixed; ++p)
+        if (((t >> p) & 1) && bank_bit(p) == 0) f = 1;
+      return kConflictCost * (f - 1);
+    }
     if (form16(ps.first_block + i)) {
       if (k4p) {
         if (!((a >> h) & 1)) return -1;
         a &= ~(1 << h);
       }
-      f = factor16[bank_bit(__builtin_ctz(unsigned(t)))][a];
+      if (k4p) {
+        // make_tile_block picks the first target
+        f = 1 << 30;
+        for (int p = 0; p < kTileBitsFixed; ++p)
+          if ((t >> p) & 1) f = std::min(f, int(factor16[bank_bit(p)][a]));
+      } else {
+        f = factor16[bank_bit(__builtin_ctz(unsigned(t)))][a];
+      }
     } else {
       f = factor[a];
     }
@@ -1456,7 +1511,7 @@ static bool layout_pass(const qcs_hip_pass &ps, const qcs_hip_block *blocks, int
     const int bi = ps.first_block + i;
     const bool last = i + 1 == nblk || wmask[i + 1] != wmask[i];
     make_tile_block(ps, blocks[bi], bi, wmask[i], nwb, last, form16(bi), whalf[i], &tb[bi],
-                    direct && (i == 0 || i + 1 == nblk));
+                    direct && (i == 0 || i + 1 == nblk), k4p);
   }
   // a k = 4 block reads the next block's first half ahead when that block is
   // also k = 4, in this run, with the same half bit

@@ -239,8 +239,10 @@ def test_tile_direct_passes_bit_identical(qlib, n, seed):
     """Direct passes of the three-product form (the first block's operands
     loaded from HBM into registers, the last block's results stored from
     registers, on passes whose first and last blocks leave qubits 0..4 to
-    the lanes) compute the same products as the LDS path: the states are
-    bit-identical with the direct form on and off.  An operand or result
+    the lanes) compute the blocks of the LDS path: the states agree to FP32
+    rounding with the direct form on and off (the two may put a different
+    target on the lanes' row bit, so the products sum in another order) and
+    stay within 1 - F <= 1e-6 of the FP64 simulation.  An operand or result
     at a wrong address, row or half would differ by O(1)."""
     from test_fusion_plan import pass_plan
     hip, _ = qlib
@@ -255,6 +257,7 @@ def test_tile_direct_passes_bit_identical(qlib, n, seed):
         B = (qcs_amd.HipBlock * len(blocks))(*blocks)
         G = (qcs_amd.HipGate * len(ordered))(*ordered)
         psi = rand_state(n, np.random.default_rng(3 * n + seed))
+        ref = HP.simulate(n, o, psi.astype(np.complex64).astype(complex))
         got = []
         for on in (1, 0):
             with qcs_amd.Circuit(n, qcs_amd.PLAIN_FP32) as st:
@@ -267,7 +270,9 @@ def test_tile_direct_passes_bit_identical(qlib, n, seed):
                                               len(blocks), G, len(ordered))
                 assert rc == 0, hip.qcs_hip_last_error()
                 got.append(st.amplitudes())
-        assert np.array_equal(got[0], got[1]), (n, i, len(direct_passes(passes, blocks)))
+        assert np.linalg.norm(got[0] - got[1]) <= 1e-5, (n, i, len(direct_passes(passes, blocks)))
+        for x in got:
+            assert 1 - abs(np.vdot(ref, x)) ** 2 / np.vdot(x, x).real <= 1e-6
     with qcs_amd.Circuit(n, qcs_amd.PLAIN_FP32) as a:
         assert hip.qcs_hip_set_tuning(a.device_state, qcs_amd.TUNE_TILE_DIRECT, 2) == -1
         v = ctypes.c_int()
