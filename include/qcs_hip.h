@@ -226,6 +226,16 @@ int qcs_hip_apply_passes(qcs_hip_state *s, const qcs_hip_pass *passes,
                          int num_passes, const qcs_hip_block *blocks,
                          int num_blocks, const qcs_hip_gate *gates,
                          int num_gates);
+/* qcs_hip_apply_passes followed by the FP64 renormalisation of
+ * qcs_hip_normalize(s, NULL) (q_state_normalize's rule, decided on the
+ * device): the fused flush.  No counterpart in the reference (its FP64 gates
+ * keep the norm).  Where the passes run in the three-product form, the last
+ * pass also sums |x|^2 of the amplitudes it writes (FP64, one partial per
+ * work-group, a fixed order), so the state is not read again for the norm. */
+int qcs_hip_apply_passes_normalize(qcs_hip_state *s, const qcs_hip_pass *passes,
+                                   int num_passes, const qcs_hip_block *blocks,
+                                   int num_blocks, const qcs_hip_gate *gates,
+                                   int num_gates);
 
 /* Launch-shape knobs of the state's kernels (for the autotuner).
  * QCS_HIP_TUNE_TILE_THREADS: 512 or 1024 threads per tile-program

@@ -107,6 +107,8 @@ _HIP_PROTOS = {
     "qcs_hip_compare": (c_int, [c_void_p, c_void_p, P(c_double), P(c_double)]),
     "qcs_hip_apply_passes": (c_int, [c_void_p, P(HipPass), c_int, P(HipBlock), c_int,
                                      P(HipGate), c_int]),
+    "qcs_hip_apply_passes_normalize": (c_int, [c_void_p, P(HipPass), c_int, P(HipBlock),
+                                               c_int, P(HipGate), c_int]),
     "qcs_hip_set_tuning": (c_int, [c_void_p, c_int, c_int]),
     "qcs_hip_get_tuning": (c_int, [c_void_p, c_int, P(c_int)]),
     "qcs_hip_timing_enable": (c_int, [c_void_p, c_int]),

@@ -572,10 +572,9 @@ int qcs_hip_apply_unitary(qcs_hip_state *s, int k, const int *targets,
   return fused_apply_unitary(s, k, targets, u);
 }
 
-int qcs_hip_apply_passes(qcs_hip_state *s, const qcs_hip_pass *passes,
-                         int num_passes, const qcs_hip_block *blocks,
-                         int num_blocks, const qcs_hip_gate *gates,
-                         int num_gates) {
+static int check_passes(qcs_hip_state *s, const qcs_hip_pass *passes, int num_passes,
+                        const qcs_hip_block *blocks, int num_blocks,
+                        const qcs_hip_gate *gates, int num_gates) {
   int rc;
   if ((rc = check_state(s))) return rc;
   if (s->flags & QCS_HIP_COMPAT)
@@ -618,8 +617,31 @@ int qcs_hip_apply_passes(qcs_hip_state *s, const qcs_hip_pass *passes,
   }
   if (covered != num_blocks)
     return fail(QCS_HIP_EINVAL, "passes cover %d of %d blocks", covered, num_blocks);
-  if ((rc = validate_blocks(s->n, blocks, num_blocks, gates, num_gates))) return rc;
+  return validate_blocks(s->n, blocks, num_blocks, gates, num_gates);
+}
+
+int qcs_hip_apply_passes(qcs_hip_state *s, const qcs_hip_pass *passes,
+                         int num_passes, const qcs_hip_block *blocks,
+                         int num_blocks, const qcs_hip_gate *gates,
+                         int num_gates) {
+  int rc;
+  if ((rc = check_passes(s, passes, num_passes, blocks, num_blocks, gates, num_gates)))
+    return rc;
   return tiles_apply(s, passes, num_passes, blocks, num_blocks, gates, num_gates);
+}
+
+int qcs_hip_apply_passes_normalize(qcs_hip_state *s, const qcs_hip_pass *passes,
+                                   int num_passes, const qcs_hip_block *blocks,
+                                   int num_blocks, const qcs_hip_gate *gates,
+                                   int num_gates) {
+  int rc, nparts = 0;
+  if ((rc = check_passes(s, passes, num_passes, blocks, num_blocks, gates, num_gates)))
+    return rc;
+  if ((rc = tiles_apply(s, passes, num_passes, blocks, num_blocks, gates, num_gates,
+                        &nparts)))
+    return rc;
+  // the last pass summed |x|^2 of what it wrote: no separate read of the state
+  return nparts > 0 ? normalize_from_partials(s, nparts) : normalize_device(s);
 }
 
 int qcs_hip_set_tuning(qcs_hip_state *s, int knob, int value) {

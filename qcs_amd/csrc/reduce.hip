@@ -717,6 +717,14 @@ int normalize_device(qcs_hip_state *s) {
              : scale_impl<float>(s, 1.0, result_slot(s));
 }
 
+int normalize_from_partials(qcs_hip_state *s, int nb) {
+  int rc;
+  if ((rc = fold(s, unsigned(nb), 1, 0u))) return rc;
+  return s->precision == QCS_HIP_C128
+             ? scale_impl<double>(s, 1.0, result_slot(s))
+             : scale_impl<float>(s, 1.0, result_slot(s));
+}
+
 int qubit_probs(qcs_hip_state *s, int q, double *p0, double *p1) {
   const unsigned nb = red_blocks(s->size);
   int rc = s->precision == QCS_HIP_C128 ? norm_partials<double>(s, q, nb)

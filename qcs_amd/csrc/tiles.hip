@@ -667,7 +667,7 @@ __device__ __forceinline__ void tp_block4g_out(const BlockScalars &d, const Bloc
                                                int nblk, int wave, BlockScalars &nd,
                                                BlockRegs &nr, const K4GState &st, float2 *a,
                                                uint64_t base, uint64_t o, uint64_t m2,
-                                               uint64_t m3) {
+                                               uint64_t m3, bool acc_norm, double &nacc) {
   static_assert((TILE >> 4) / 32 / (NTH / 64) == 1, "one 32-column group per wave");
   const i32x2 nl = load_lanes4g(nblk);
   __builtin_amdgcn_sched_barrier(0);
@@ -685,6 +685,15 @@ __device__ __forceinline__ void tp_block4g_out(const BlockScalars &d, const Bloc
   for (int v = 0; v < 4; ++v)
     st_nt(a + (base | o | ((v & 1) ? m2 : 0) | ((v & 2) ? m3 : 0)),
           make_float4(re0[v], -nim0[v], re1[v], -nim1[v]));
+  if (acc_norm) {
+    // |x|^2 of the values written, in FP64, in a fixed order
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const double x0 = re0[v], y0 = nim0[v], x1 = re1[v], y1 = nim1[v];
+      nacc = nacc + (x0 * x0 + y0 * y0);
+      nacc = nacc + (x1 * x1 + y1 * y1);
+    }
+  }
 }
 
 // One FP32 block of the pass (index b), with the next block's values read
@@ -777,6 +786,22 @@ __device__ __forceinline__ void tp_block_split(const TileBlock *__restrict__ bl,
   }
 }
 
+// The work-group's FP64 sum of v to norm[blockIdx.x * kRedSlots]: lanes by
+// the wave's fixed xor tree, then the waves in order (the tile's LDS is free
+// once its last tile is written)
+template <int NTH>
+__device__ __forceinline__ void wg_norm_out(double v, double *norm) {
+  v = wave_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) lds_store<double>(8 * int(threadIdx.x >> 6), v);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < NTH / 64; ++w) t = t + lds_load<double>(8 * w);
+    norm[size_t(blockIdx.x) * kRedSlots] = t;
+  }
+}
+
 // a kernel row pattern as a row of the composed unitary (TileBlock::tidx)
 __device__ __forceinline__ int sigma4(const TileBlock &bl, int r) {
   int x = 0;
@@ -799,7 +824,7 @@ template <int TB, int NTH, bool SPLIT, int K4>
 __global__ __launch_bounds__(NTH) void k_tile_program(
     float2 *__restrict__ a, const TilePass *__restrict__ passes, int pass_index,
     const TileBlock *__restrict__ blocks, const float *__restrict__ wbuf,
-    uint64_t ntiles) {
+    uint64_t ntiles, double *__restrict__ norm) {
   constexpr int TILE = 1 << TB;
   constexpr bool DIR = K4 == 3;
   constexpr int KF = DIR ? 2 : K4;  // the block form
@@ -961,6 +986,7 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
     BlockRegs ra;
     load_regs4g(ra, OPS);
     const TileBlock *fb = &blocks[ps.first_block];
+    double nacc = 0.0;  // the pass's |x|^2 when `norm` is given
     auto step = [&](float4 (&cur)[4]) {
       K4GState st;
 #pragma unroll
@@ -977,7 +1003,7 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
         sa = sb;
       }
       tp_block4g_out<NTH, TILE>(sa, ra, nb == 1 ? 1.0f : -1.0f, fb, OPS, wave, sb, ra, st, a,
-                                base, lo, lr2, lr3);
+                                base, lo, lr2, lr3, norm != nullptr, nacc);
       sa = sb;
       // the next tile's first block writes the LDS tile its last block read
       __syncthreads();
@@ -991,6 +1017,7 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
       if (tile >= tend) break;
       step(xb);
     }
+    if (norm) wg_norm_out<NTH>(nacc, norm);
     return;
   }
 
@@ -1041,6 +1068,7 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
     for (int it = 0; it < IT; ++it) v0[it] = ld_nt(a + (lb | it_addr(it) | addr_t));
   }
 
+  double nacc = 0.0;  // the pass's |x|^2 when `norm` is given
   // one tile: load tile + DEPTH into `ld`, apply the blocks, write the tile
   // back and move tile + 1 from `fl` into LDS
   auto tile_step = [&](float4 (&ld)[IT], float4 (&fl)[IT]) {
@@ -1125,6 +1153,15 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
       else
         o[it] = make_float4(re[s0], im[s0], re[s0 ^ 1], im[s0 ^ 1]);
     }
+    if (norm) {
+      // |x|^2 of the values written, in FP64, in a fixed order
+#pragma unroll
+      for (int it = 0; it < IT; ++it) {
+        const double x0 = o[it].x, y0 = o[it].y, x1 = o[it].z, y1 = o[it].w;
+        nacc = nacc + (x0 * x0 + y0 * y0);
+        nacc = nacc + (x1 * x1 + y1 * y1);
+      }
+    }
     // each thread refills the slots it has just read out: no other thread
     // touches them in between, so no barrier is needed here
     if (next < tend) {
@@ -1156,6 +1193,7 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
   } else {
     while (tile < tend) tile_step(v0, v0);
   }
+  if (norm) wg_norm_out<NTH>(nacc, norm);
 }
 
 // ---- host side ------------------------------------------------------------------
@@ -1172,7 +1210,8 @@ static int grow_dev(void **p, size_t *cap, size_t need) {
 
 template <int NTH, bool SPLIT, int K4>
 static int launch_tp(qcs_hip_state *s, const TilePass *dpass, int pi,
-                     const TileBlock *dblocks, uint64_t ntiles, int nblocks) {
+                     const TileBlock *dblocks, uint64_t ntiles, int nblocks,
+                     double *norm = nullptr, int *grid_out = nullptr) {
   // once per kernel instance (states may launch from separate threads)
   static std::once_flag once;
   static int attr_rc = 0;
@@ -1202,9 +1241,12 @@ static int launch_tp(qcs_hip_state *s, const TilePass *dpass, int pi,
   const uint64_t per_cu = kLdsPerCu / lds;  // work-groups that fit per CU
   uint64_t grid = uint64_t(s->tile_grid) * (per_cu > 1 ? 2 : 1);
   if (grid > ntiles) grid = ntiles;
+  // the pass's norm: one partial per work-group in the reduction buffer
+  if (norm && grid > uint64_t(kRedBlocks)) norm = nullptr;
+  if (grid_out) *grid_out = norm ? int(grid) : 0;
   hipLaunchKernelGGL((k_tile_program<kTileBitsFixed, NTH, SPLIT, K4>), dim3(unsigned(grid)),
                      dim3(NTH), lds, s->stream, static_cast<float2 *>(s->vec),
-                     dpass, pi, dblocks, s->wbuf, ntiles);
+                     dpass, pi, dblocks, s->wbuf, ntiles, norm);
   return launch_check("k_tile_program");
 }
 
@@ -1555,7 +1597,8 @@ static void pad_blocks(const qcs_hip_pass *passes, int np, const qcs_hip_block *
 
 int tiles_apply(qcs_hip_state *s, const qcs_hip_pass *passes, int np,
                 const qcs_hip_block *blocks, int nb, const qcs_hip_gate *gates,
-                int ng) {
+                int ng, int *norm_partials) {
+  if (norm_partials) *norm_partials = 0;
   if (np <= 0) return 0;
   const size_t pbytes = sizeof(TilePass) * size_t(np);
   const size_t tbytes = sizeof(TileBlock) * size_t(nb > 0 ? nb : 1);
@@ -1609,6 +1652,9 @@ int tiles_apply(qcs_hip_state *s, const qcs_hip_pass *passes, int np,
   const TileBlock *db = static_cast<const TileBlock *>(s->tblock_dev);
   const bool split = s->tile_math == QCS_HIP_TILE_MATH_SPLIT_BF16;
   const double N = double(s->size);
+  int last_pass = -1;
+  for (int p = 0; p < np; ++p)
+    if (passes[p].num_blocks > 0) last_pass = p;
   for (int p = 0; p < np; ++p) {
     // the kernel runs its pass's first block unconditionally: an empty pass
     // (which the planner does not make) would change nothing anyway
@@ -1621,12 +1667,16 @@ int tiles_apply(qcs_hip_state *s, const qcs_hip_pass *passes, int np,
     TimedLaunch tl;
     timing_begin(s, QCS_HIP_K_FUSED, 2.0 * N * double(s->amp_bytes), flops, &tl);
     const int nbp = passes[p].num_blocks;
+    // the last pass of the three-product kernel also sums |x|^2 of what it
+    // writes, one FP64 partial per work-group, when the caller asks for it
+    double *nrm = norm_partials && p == last_pass ? s->red : nullptr;
     if (s->tile_threads == 1024)
       rc = split ? launch_tp<1024, true, 0>(s, dp, p, db, ntiles, nbp)
            : !k4p ? launch_tp<1024, false, 0>(s, dp, p, db, ntiles, nbp)
            : s->tile_k4 == QCS_HIP_TILE_K4_GAUSS
-               ? (tp[p].direct ? launch_tp<1024, false, 3>(s, dp, p, db, ntiles, nbp)
-                               : launch_tp<1024, false, 2>(s, dp, p, db, ntiles, nbp))
+               ? (tp[p].direct
+                      ? launch_tp<1024, false, 3>(s, dp, p, db, ntiles, nbp, nrm, norm_partials)
+                      : launch_tp<1024, false, 2>(s, dp, p, db, ntiles, nbp, nrm, norm_partials))
                : launch_tp<1024, false, 1>(s, dp, p, db, ntiles, nbp);
     else
       rc = split ? launch_tp<512, true, 0>(s, dp, p, db, ntiles, nbp)

@@ -193,19 +193,19 @@ static void flush_pending(t_q_circuit *c) {
     fprintf(stderr, "Error: fusion planning failed.\n");
     c->error = QCS_HIP_ENOMEM;
   } else {
+    /* then the FP64 norm reduction + renormalisation, decided on the device
+       (tile programs: the norm summed by their last pass) */
     if (passes)
-      rc = check(c, qcs_hip_apply_passes(c->state, passes, np, blocks, nb,
-                                         ordered, c->pending_size),
+      rc = check(c, qcs_hip_apply_passes_normalize(c->state, passes, np, blocks, nb,
+                                                   ordered, c->pending_size),
                  "tile programs");
-    else
-      rc = check(c, qcs_hip_apply_fused(c->state, blocks, nb, ordered,
-                                        c->pending_size),
-                 "fused apply");
+    else if ((rc = check(c, qcs_hip_apply_fused(c->state, blocks, nb, ordered,
+                                                c->pending_size),
+                         "fused apply")) == 0)
+      rc = check(c, qcs_hip_normalize(c->state, NULL), "renormalise");
     if (rc == 0) {
       c->fused_blocks += nb;
       c->fused_passes += passes ? np : nb;
-      /* FP64 norm reduction + renormalisation, decided on the device */
-      check(c, qcs_hip_normalize(c->state, NULL), "renormalise");
     }
   }
   free(passes);

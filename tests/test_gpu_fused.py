@@ -280,6 +280,52 @@ def test_tile_direct_passes_bit_identical(qlib, n, seed):
                                       ctypes.byref(v)) == 0 and v.value == 1
 
 
+@pytest.mark.parametrize("n,seed", [(16, 4), (20, 5), (22, 6)])
+def test_apply_passes_normalize_equals_apply_then_normalize(qlib, n, seed):
+    """qcs_hip_apply_passes_normalize (the last three-product pass sums |x|^2
+    of what it writes) leaves the state of qcs_hip_apply_passes followed by
+    qcs_hip_normalize: the two norms are the same FP64 sum in another order,
+    so the states agree to FP32 rounding, and a state of norm^2 2.89 comes
+    out of norm 1.  Covered: the last pass direct or through LDS, the chain
+    form (no partials: the separate norm pass), a zero state (left as it is,
+    q_state_normalize's norm_sq > 1e-12 rule)."""
+    from test_fusion_plan import pass_plan
+    hip, _ = qlib
+    print(f"[fused] apply+normalize n={n}", flush=True)
+    o = C.layered(n, 5, seed)
+    passes, blocks, ordered = pass_plan(o, n)
+    P = (qcs_amd.HipPass * len(passes))(*passes)
+    B = (qcs_amd.HipBlock * len(blocks))(*blocks)
+    G = (qcs_amd.HipGate * len(ordered))(*ordered)
+    psi = 1.7 * rand_state(n, np.random.default_rng(5 * n + seed))
+    for form, direct in ((qcs_amd.TILE_K4_GAUSS, 1), (qcs_amd.TILE_K4_GAUSS, 0),
+                         (qcs_amd.TILE_K4_CHAIN, 1)):
+        got = []
+        for fused in (True, False):
+            with qcs_amd.Circuit(n, qcs_amd.PLAIN_FP32) as st:
+                st.set_amplitudes(psi)
+                assert hip.qcs_hip_set_tuning(st.device_state, qcs_amd.TUNE_TILE_K4_FORM,
+                                              form) == 0
+                assert hip.qcs_hip_set_tuning(st.device_state, qcs_amd.TUNE_TILE_DIRECT,
+                                              direct) == 0
+                if fused:
+                    rc = hip.qcs_hip_apply_passes_normalize(st.device_state, P, len(passes),
+                                                            B, len(blocks), G, len(ordered))
+                else:
+                    rc = hip.qcs_hip_apply_passes(st.device_state, P, len(passes), B,
+                                                  len(blocks), G, len(ordered))
+                    rc = rc or hip.qcs_hip_normalize(st.device_state, None)
+                assert rc == 0, hip.qcs_hip_last_error()
+                got.append(st.amplitudes())
+        assert abs(np.vdot(got[0], got[0]).real - 1.0) <= 1e-5, (form, direct)
+        assert np.linalg.norm(got[0] - got[1]) <= 1e-6, (form, direct)
+    with qcs_amd.Circuit(n, qcs_amd.PLAIN_FP32) as st:
+        st.set_amplitudes(np.zeros(1 << n, complex))
+        assert hip.qcs_hip_apply_passes_normalize(st.device_state, P, len(passes), B,
+                                                  len(blocks), G, len(ordered)) == 0
+        assert not np.any(st.amplitudes())
+
+
 def test_apply_passes_rejects_bad_plans(qlib):
     from test_fusion_plan import pass_plan
     hip, _ = qlib
@@ -288,19 +334,20 @@ def test_apply_passes_rejects_bad_plans(qlib):
     passes, blocks, ordered = pass_plan(ops, n)
     B = (qcs_amd.HipBlock * len(blocks))(*blocks)
     G = (qcs_amd.HipGate * len(ordered))(*ordered)
-    with qcs_amd.Circuit(n, qcs_amd.PLAIN_FP32) as a:
-        bad = (qcs_amd.HipPass * len(passes))(*passes)
-        bad[0].bits[0] = 1  # tile must start with bits 0..5
-        assert hip.qcs_hip_apply_passes(a.device_state, bad, len(passes), B,
-                                        len(blocks), G, len(ordered)) == -1
-        bad = (qcs_amd.HipPass * len(passes))(*passes)
-        bad[0].num_blocks += 1  # blocks not covered exactly once
-        assert hip.qcs_hip_apply_passes(a.device_state, bad, len(passes), B,
-                                        len(blocks), G, len(ordered)) == -1
-    with qcs_amd.Circuit(n) as c128:
-        good = (qcs_amd.HipPass * len(passes))(*passes)
-        assert hip.qcs_hip_apply_passes(c128.device_state, good, len(passes), B,
-                                        len(blocks), G, len(ordered)) == -5
+    for entry in (hip.qcs_hip_apply_passes, hip.qcs_hip_apply_passes_normalize):
+        with qcs_amd.Circuit(n, qcs_amd.PLAIN_FP32) as a:
+            bad = (qcs_amd.HipPass * len(passes))(*passes)
+            bad[0].bits[0] = 1  # tile must start with bits 0..5
+            assert entry(a.device_state, bad, len(passes), B, len(blocks), G,
+                         len(ordered)) == -1
+            bad = (qcs_amd.HipPass * len(passes))(*passes)
+            bad[0].num_blocks += 1  # blocks not covered exactly once
+            assert entry(a.device_state, bad, len(passes), B, len(blocks), G,
+                         len(ordered)) == -1
+        with qcs_amd.Circuit(n) as c128:
+            good = (qcs_amd.HipPass * len(passes))(*passes)
+            assert entry(c128.device_state, good, len(passes), B, len(blocks), G,
+                         len(ordered)) == -5
 
 
 @pytest.mark.parametrize("n,depth", [(16, 6), (20, 8), (24, 4)])
