@@ -1201,6 +1201,13 @@ def main():
             "k_cu_gbs": (b2 / max(n2, 1)) / (m2 / max(n2, 1) * 1e-3) / 1e9 if n2 else None,
         }
         result["plain_fp64"]["k_1q_hbm_frac"] = (result["plain_fp64"]["k_1q_gbs"] or 0) / HBM_PEAK_GBS
+        # the plain form as a whole: algorithmic bytes of every gate over the
+        # gate kernels' time, and over the step's wall time
+        if n1 + n2:
+            gk = (b1 + b2) / ((m1 + m2) * 1e-3) / 1e9
+            result["plain_fp64"]["gate_kernels_gbs"] = gk
+            result["plain_fp64"]["gate_kernels_hbm_frac"] = gk / HBM_PEAK_GBS
+            result["plain_fp64"]["step_hbm_frac"] = (b1 + b2) / Tp / 1e9 / HBM_PEAK_GBS
         # what HBM moved: whole 128-byte lines (a controlled gate on qubit 0,
         # 1 or 2 reads and writes every line, twice its algorithmic bytes)
         for key, mv, ms_, n_ in (("k_1q", mv1, m1, n1), ("k_cu", mv2, m2, n2)):
