@@ -226,12 +226,17 @@ int qcs_hip_apply_passes(qcs_hip_state *s, const qcs_hip_pass *passes,
                          int num_passes, const qcs_hip_block *blocks,
                          int num_blocks, const qcs_hip_gate *gates,
                          int num_gates);
-/* qcs_hip_apply_passes followed by the FP64 renormalisation of
- * qcs_hip_normalize(s, NULL) (q_state_normalize's rule, decided on the
- * device): the fused flush.  No counterpart in the reference (its FP64 gates
- * keep the norm).  Where the passes run in the three-product form, the last
- * pass also sums |x|^2 of the amplitudes it writes (FP64, one partial per
- * work-group, a fixed order), so the state is not read again for the norm. */
+/* qcs_hip_apply_passes with the FP64 renormalisation of qcs_hip_normalize
+ * (q_state_normalize's rule, decided on the device): the fused flush.  No
+ * counterpart in the reference (its FP64 gates keep the norm).  In the
+ * three-product form with two passes or more, the pass before the last sums
+ * |x|^2 of the amplitudes it writes (FP64, one partial per work-group, a
+ * fixed order) and the last pass scales its first block by 1/sqrt(norm_sq):
+ * the state is renormalised ahead of the last pass, its norm after it is 1
+ * to that pass's FP32 rounding, and no kernel reads the state only for the
+ * norm or the rescale.  With one such pass, it sums the norm of what it
+ * writes and the rescale follows; in the other forms
+ * qcs_hip_normalize(s, NULL) follows the passes. */
 int qcs_hip_apply_passes_normalize(qcs_hip_state *s, const qcs_hip_pass *passes,
                                    int num_passes, const qcs_hip_block *blocks,
                                    int num_blocks, const qcs_hip_gate *gates,

@@ -640,6 +640,7 @@ int qcs_hip_apply_passes_normalize(qcs_hip_state *s, const qcs_hip_pass *passes,
   if ((rc = tiles_apply(s, passes, num_passes, blocks, num_blocks, gates, num_gates,
                         &nparts)))
     return rc;
+  if (nparts < 0) return 0;  // renormalised by the last pass
   // the last pass summed |x|^2 of what it wrote: no separate read of the state
   return nparts > 0 ? normalize_from_partials(s, nparts) : normalize_device(s);
 }

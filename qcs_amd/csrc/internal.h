@@ -172,14 +172,19 @@ int tile_max_blocks(const qcs_hip_state *s);
 int launch_block_mfma(qcs_hip_state *s, int k, const int *targets,
                       const float *W, int variant);
 int block_variant_for(const qcs_hip_state *s, int variant);
-// norm_partials (may be null): on return the number of FP64 partial sums of
-// |x|^2 the last pass left in s->red (one per work-group), 0 when it did not
+// norm_partials (may be null): on return -1 when the passes renormalised the
+// state themselves (the last pass scaled by the norm of its input, under
+// q_state_normalize's rule), else the number of FP64 partial sums of |x|^2
+// the last pass left in s->red (one per work-group), 0 when it did not
 int tiles_apply(qcs_hip_state *s, const qcs_hip_pass *passes, int np,
                 const qcs_hip_block *blocks, int nb, const qcs_hip_gate *gates,
                 int ng, int *norm_partials = nullptr);
 // renormalise from `nb` partial sums already in s->red (FP64, the rule of
 // normalize_device)
 int normalize_from_partials(qcs_hip_state *s, int nb);
+// fold `nb` partial sums in s->red into the device result slot; *ns_dev
+// points at it (the FP64 norm_sq, valid in stream order)
+int fold_norm_partials(qcs_hip_state *s, int nb, const double **ns_dev);
 
 __host__ __device__ inline uint64_t ceil_div(uint64_t a, uint64_t b) {
   return (a + b - 1) / b;

@@ -717,6 +717,11 @@ int normalize_device(qcs_hip_state *s) {
              : scale_impl<float>(s, 1.0, result_slot(s));
 }
 
+int fold_norm_partials(qcs_hip_state *s, int nb, const double **ns_dev) {
+  *ns_dev = result_slot(s);
+  return fold(s, unsigned(nb), 1, 0u);
+}
+
 int normalize_from_partials(qcs_hip_state *s, int nb) {
   int rc;
   if ((rc = fold(s, unsigned(nb), 1, 0u))) return rc;

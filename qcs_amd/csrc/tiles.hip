@@ -824,7 +824,7 @@ template <int TB, int NTH, bool SPLIT, int K4>
 __global__ __launch_bounds__(NTH) void k_tile_program(
     float2 *__restrict__ a, const TilePass *__restrict__ passes, int pass_index,
     const TileBlock *__restrict__ blocks, const float *__restrict__ wbuf,
-    uint64_t ntiles, double *__restrict__ norm) {
+    uint64_t ntiles, double *__restrict__ norm, const double *__restrict__ fsum) {
   constexpr int TILE = 1 << TB;
   constexpr bool DIR = K4 == 3;
   constexpr int KF = DIR ? 2 : K4;  // the block form
@@ -844,6 +844,14 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
   const int tid = threadIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 
+  // fsum (three-product form only): the FP64 norm_sq of the pass's input;
+  // the first block's products are scaled by 1/sqrt(norm_sq) under
+  // q_state_normalize's rule, so the pass also renormalises
+  double fscale = 1.0;
+  if (!SPLIT && KF == 2 && fsum) {
+    const double v = *fsum;
+    if (v > 1e-12 && v != 1.0) fscale = 1.0 / sqrt(v);
+  }
   // stage the A operands and lane tables of every block
   for (int idx = tid; idx < nb * 64 * 16; idx += NTH) {
     const int b = idx / (64 * 16), ln = (idx / 16) % 64, s = idx % 16;
@@ -866,6 +874,7 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
             : p == 0 ? W[RR * RR + ro * R + si]
             : p == 1 ? (b == 0 ? -1.0f : 1.0f) * W[RR * RR + R * R + ro * R + si]
                      : -W[(2 * ro + 1) * RR + 2 * si];
+        if (b == 0) w = float(double(w) * fscale);
       } else {
         const int lrows = ((ln & 16) ? bl.rowoff[1] : 0) ^ ((ln & 32) ? bl.rowoff[2] : 0);
         w = t < 2 ? __int_as_float(bl.lanecol[(ln & 15) | (t << 4)] ^ lrows) : 0.0f;
@@ -1211,7 +1220,8 @@ static int grow_dev(void **p, size_t *cap, size_t need) {
 template <int NTH, bool SPLIT, int K4>
 static int launch_tp(qcs_hip_state *s, const TilePass *dpass, int pi,
                      const TileBlock *dblocks, uint64_t ntiles, int nblocks,
-                     double *norm = nullptr, int *grid_out = nullptr) {
+                     double *norm = nullptr, int *grid_out = nullptr,
+                     const double *fsum = nullptr) {
   // once per kernel instance (states may launch from separate threads)
   static std::once_flag once;
   static int attr_rc = 0;
@@ -1246,7 +1256,7 @@ static int launch_tp(qcs_hip_state *s, const TilePass *dpass, int pi,
   if (grid_out) *grid_out = norm ? int(grid) : 0;
   hipLaunchKernelGGL((k_tile_program<kTileBitsFixed, NTH, SPLIT, K4>), dim3(unsigned(grid)),
                      dim3(NTH), lds, s->stream, static_cast<float2 *>(s->vec),
-                     dpass, pi, dblocks, s->wbuf, ntiles, norm);
+                     dpass, pi, dblocks, s->wbuf, ntiles, norm, fsum);
   return launch_check("k_tile_program");
 }
 
@@ -1652,9 +1662,22 @@ int tiles_apply(qcs_hip_state *s, const qcs_hip_pass *passes, int np,
   const TileBlock *db = static_cast<const TileBlock *>(s->tblock_dev);
   const bool split = s->tile_math == QCS_HIP_TILE_MATH_SPLIT_BF16;
   const double N = double(s->size);
-  int last_pass = -1;
+  int last_pass = -1, prev_pass = -1;
   for (int p = 0; p < np; ++p)
-    if (passes[p].num_blocks > 0) last_pass = p;
+    if (passes[p].num_blocks > 0) {
+      prev_pass = last_pass;
+      last_pass = p;
+    }
+  // Renormalisation (when the caller asks for the norm).  In the
+  // three-product form with two passes or more, the pass before the last
+  // sums |x|^2 of what it writes (one FP64 partial per work-group), the
+  // partials are folded on the device, and the last pass scales its first
+  // block by 1/sqrt(norm_sq): no pass of its own reads the state to rescale
+  // it.  Otherwise the last pass sums the norm of its output for the caller.
+  const bool gauss = !split && k4p && s->tile_k4 == QCS_HIP_TILE_K4_GAUSS &&
+                     s->tile_threads == 1024;
+  const bool scale_ahead = norm_partials && gauss && prev_pass >= 0;
+  const double *ns_dev = nullptr;  // the folded norm_sq, once available
   for (int p = 0; p < np; ++p) {
     // the kernel runs its pass's first block unconditionally: an empty pass
     // (which the planner does not make) would change nothing anyway
@@ -1667,22 +1690,32 @@ int tiles_apply(qcs_hip_state *s, const qcs_hip_pass *passes, int np,
     TimedLaunch tl;
     timing_begin(s, QCS_HIP_K_FUSED, 2.0 * N * double(s->amp_bytes), flops, &tl);
     const int nbp = passes[p].num_blocks;
-    // the last pass of the three-product kernel also sums |x|^2 of what it
-    // writes, one FP64 partial per work-group, when the caller asks for it
-    double *nrm = norm_partials && p == last_pass ? s->red : nullptr;
+    double *nrm = nullptr;
+    const double *fs = nullptr;
+    if (norm_partials && p == last_pass) {
+      if (ns_dev) fs = ns_dev;
+      else nrm = s->red;
+    } else if (scale_ahead && p == prev_pass) {
+      nrm = s->red;
+    }
+    int parts = 0;
     if (s->tile_threads == 1024)
       rc = split ? launch_tp<1024, true, 0>(s, dp, p, db, ntiles, nbp)
            : !k4p ? launch_tp<1024, false, 0>(s, dp, p, db, ntiles, nbp)
            : s->tile_k4 == QCS_HIP_TILE_K4_GAUSS
                ? (tp[p].direct
-                      ? launch_tp<1024, false, 3>(s, dp, p, db, ntiles, nbp, nrm, norm_partials)
-                      : launch_tp<1024, false, 2>(s, dp, p, db, ntiles, nbp, nrm, norm_partials))
+                      ? launch_tp<1024, false, 3>(s, dp, p, db, ntiles, nbp, nrm, &parts, fs)
+                      : launch_tp<1024, false, 2>(s, dp, p, db, ntiles, nbp, nrm, &parts, fs))
                : launch_tp<1024, false, 1>(s, dp, p, db, ntiles, nbp);
     else
       rc = split ? launch_tp<512, true, 0>(s, dp, p, db, ntiles, nbp)
                  : launch_tp<512, false, 0>(s, dp, p, db, ntiles, nbp);
     if (rc) return rc;
     if ((rc = timing_end(s, &tl))) return rc;
+    if (norm_partials && p == last_pass) *norm_partials = fs ? -1 : parts;
+    if (scale_ahead && p == prev_pass && parts > 0 &&
+        (rc = fold_norm_partials(s, parts, &ns_dev)))
+      return rc;
   }
   // the staging buffer holds the descriptors until the copies have run
   QCS_HIP_TRY(hipStreamSynchronize(s->stream));

@@ -282,24 +282,32 @@ def test_tile_direct_passes_bit_identical(qlib, n, seed):
 
 @pytest.mark.parametrize("n,seed", [(16, 4), (20, 5), (22, 6)])
 def test_apply_passes_normalize_equals_apply_then_normalize(qlib, n, seed):
-    """qcs_hip_apply_passes_normalize (the last three-product pass sums |x|^2
-    of what it writes) leaves the state of qcs_hip_apply_passes followed by
-    qcs_hip_normalize: the two norms are the same FP64 sum in another order,
-    so the states agree to FP32 rounding, and a state of norm^2 2.89 comes
-    out of norm 1.  Covered: the last pass direct or through LDS, the chain
-    form (no partials: the separate norm pass), a zero state (left as it is,
-    q_state_normalize's norm_sq > 1e-12 rule)."""
+    """qcs_hip_apply_passes_normalize (three-product form: the pass before
+    the last sums |x|^2 of what it writes and the last pass scales its first
+    block by 1/sqrt(norm_sq)) leaves the state of qcs_hip_apply_passes
+    followed by qcs_hip_normalize: the two scale factors differ by the norm
+    change of one FP32 pass (~1e-7), so the states agree to 1e-6, and a state
+    of norm^2 2.89 comes out of norm 1 within 1e-5.  Covered: the last passes
+    direct or through LDS, a one-pass plan (the last pass sums the norm, the
+    rescale follows), the chain form (the separate norm and rescale), a zero
+    state (left as it is, q_state_normalize's norm_sq > 1e-12 rule)."""
     from test_fusion_plan import pass_plan
     hip, _ = qlib
     print(f"[fused] apply+normalize n={n}", flush=True)
     o = C.layered(n, 5, seed)
     passes, blocks, ordered = pass_plan(o, n)
-    P = (qcs_amd.HipPass * len(passes))(*passes)
-    B = (qcs_amd.HipBlock * len(blocks))(*blocks)
-    G = (qcs_amd.HipGate * len(ordered))(*ordered)
+    assert len(passes) >= 2
     psi = 1.7 * rand_state(n, np.random.default_rng(5 * n + seed))
-    for form, direct in ((qcs_amd.TILE_K4_GAUSS, 1), (qcs_amd.TILE_K4_GAUSS, 0),
-                         (qcs_amd.TILE_K4_CHAIN, 1)):
+    for form, direct, npass in ((qcs_amd.TILE_K4_GAUSS, 1, len(passes)),
+                                (qcs_amd.TILE_K4_GAUSS, 0, len(passes)),
+                                (qcs_amd.TILE_K4_GAUSS, 1, 1),
+                                (qcs_amd.TILE_K4_CHAIN, 1, len(passes))):
+        # a one-pass plan: the first pass's blocks only
+        nblk = sum(p.num_blocks for p in passes[:npass])
+        ngate = sum(b.num_gates for b in blocks[:nblk])
+        P = (qcs_amd.HipPass * npass)(*passes[:npass])
+        B = (qcs_amd.HipBlock * nblk)(*blocks[:nblk])
+        G = (qcs_amd.HipGate * ngate)(*ordered[:ngate])
         got = []
         for fused in (True, False):
             with qcs_amd.Circuit(n, qcs_amd.PLAIN_FP32) as st:
@@ -309,16 +317,19 @@ def test_apply_passes_normalize_equals_apply_then_normalize(qlib, n, seed):
                 assert hip.qcs_hip_set_tuning(st.device_state, qcs_amd.TUNE_TILE_DIRECT,
                                               direct) == 0
                 if fused:
-                    rc = hip.qcs_hip_apply_passes_normalize(st.device_state, P, len(passes),
-                                                            B, len(blocks), G, len(ordered))
+                    rc = hip.qcs_hip_apply_passes_normalize(st.device_state, P, npass,
+                                                            B, nblk, G, ngate)
                 else:
-                    rc = hip.qcs_hip_apply_passes(st.device_state, P, len(passes), B,
-                                                  len(blocks), G, len(ordered))
+                    rc = hip.qcs_hip_apply_passes(st.device_state, P, npass, B, nblk, G,
+                                                  ngate)
                     rc = rc or hip.qcs_hip_normalize(st.device_state, None)
                 assert rc == 0, hip.qcs_hip_last_error()
                 got.append(st.amplitudes())
-        assert abs(np.vdot(got[0], got[0]).real - 1.0) <= 1e-5, (form, direct)
-        assert np.linalg.norm(got[0] - got[1]) <= 1e-6, (form, direct)
+        assert abs(np.vdot(got[0], got[0]).real - 1.0) <= 1e-5, (form, direct, npass)
+        assert np.linalg.norm(got[0] - got[1]) <= 1e-6, (form, direct, npass)
+    P = (qcs_amd.HipPass * len(passes))(*passes)
+    B = (qcs_amd.HipBlock * len(blocks))(*blocks)
+    G = (qcs_amd.HipGate * len(ordered))(*ordered)
     with qcs_amd.Circuit(n, qcs_amd.PLAIN_FP32) as st:
         st.set_amplitudes(np.zeros(1 << n, complex))
         assert hip.qcs_hip_apply_passes_normalize(st.device_state, P, len(passes), B,
