@@ -38,6 +38,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <new>
+#include <atomic>
 #include <thread>
 #include <vector>
 
@@ -1030,6 +1031,15 @@ struct BatchLayout {
   size_t circ, order, sync, pass, gate, group, block, hgate, hblock, u, total;
 };
 
+// one host thread's share of a run's device ops (gates, register groups,
+// blocks) and the passes it built
+struct BatchPart {
+  std::vector<BGate> g;
+  std::vector<BGroup> grp;
+  std::vector<BBlock> blk;
+  std::vector<int> own;
+};
+
 }  // namespace qcs
 
 struct qcs_hip_batch {
@@ -1056,6 +1066,7 @@ struct qcs_hip_batch {
   long *h_out = nullptr;
   size_t hout_cap = 0;
   long generation = 0;  // bumped whenever a buffer moves
+  std::vector<qcs::BatchPart> parts;  // host op lists, reused between runs
   // captured launch sequence
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -1118,6 +1129,78 @@ static int max_tb(int prec) {
 }
 
 // every argument check of qcs_hip_batch_run; fills the op counts
+static int host_threads() {
+  int t = int(std::thread::hardware_concurrency());
+  const char *env = std::getenv("QCS_HOST_THREADS");
+  if (env && std::atoi(env) > 0) t = std::atoi(env);
+  return std::max(1, std::min(t, 64));
+}
+
+// circuit i of a batch: qubits, precision, tile size, its passes and their
+// gates or blocks (0, or QCS_HIP_EINVAL with this thread's message set)
+static int validate_circuit(int i, const qcs_hip_batch_circuit *cs,
+                            const qcs_hip_pass *passes, int np,
+                            const qcs_hip_block *blocks, int nb,
+                            const qcs_hip_gate *gates, int ng) {
+  const qcs_hip_batch_circuit &c = cs[i];
+  if (c.num_qubits < 1 || c.num_qubits > kBMaxN)
+    return fail(QCS_HIP_EINVAL, "circuit %d: num_qubits %d outside [1,%d]", i,
+                c.num_qubits, kBMaxN);
+  if (c.precision != QCS_HIP_C128 && c.precision != QCS_HIP_C64)
+    return fail(QCS_HIP_EINVAL, "circuit %d: unknown precision %d", i, c.precision);
+  const int n = c.num_qubits, tb = c.tile_bits;
+  if (tb < 1 || tb > n || tb > max_tb(c.precision))
+    return fail(QCS_HIP_EINVAL, "circuit %d: tile_bits %d outside [1,%d]", i, tb,
+                std::min(n, max_tb(c.precision)));
+  if (c.precision == QCS_HIP_C64 && tb < 5 && tb != n)
+    return fail(QCS_HIP_EINVAL, "circuit %d: complex64 tiles of < 5 bits must hold the state", i);
+  if (c.num_passes < 1 || c.first_pass < 0 || c.first_pass + c.num_passes > np)
+    return fail(QCS_HIP_EINVAL, "circuit %d: pass range outside the pass array", i);
+  for (int p = c.first_pass; p < c.first_pass + c.num_passes; ++p) {
+    const qcs_hip_pass &ps = passes[p];
+    if (ps.tile_bits != tb)
+      return fail(QCS_HIP_EINVAL, "pass %d: tile_bits %d != circuit's %d", p,
+                  ps.tile_bits, tb);
+    for (int j = 0; j < tb; ++j) {
+      if (ps.bits[j] < 0 || ps.bits[j] >= n || (j > 0 && ps.bits[j] <= ps.bits[j - 1]))
+        return fail(QCS_HIP_EINVAL, "pass %d: bits not ascending in range", p);
+    }
+    if (c.precision == QCS_HIP_C64 && ps.bits[0] != 0)
+      return fail(QCS_HIP_EINVAL, "pass %d: complex64 tiles must hold bit 0", p);
+    unsigned long long mask = 0;
+    for (int j = 0; j < tb; ++j) mask |= 1ull << ps.bits[j];
+    if (ps.first_block < 0 || ps.num_blocks < 0)
+      return fail(QCS_HIP_EINVAL, "pass %d: bad op range", p);
+    if (c.precision == QCS_HIP_C128) {
+      if (ps.first_block + ps.num_blocks > ng)
+        return fail(QCS_HIP_EINVAL, "pass %d: gate range outside the gate array", p);
+      for (int g = ps.first_block; g < ps.first_block + ps.num_blocks; ++g) {
+        const qcs_hip_gate &gt = gates[g];
+        if (gt.kind < QCS_HIP_G_1Q || gt.kind > QCS_HIP_G_SWAP || gt.target < 0 ||
+            gt.target >= n || !((mask >> gt.target) & 1))
+          return fail(QCS_HIP_EINVAL, "pass %d: gate %d outside the pass bits", p, g);
+        if (gt.kind != QCS_HIP_G_1Q &&
+            (gt.control < 0 || gt.control >= n || gt.control == gt.target ||
+             !((mask >> gt.control) & 1)))
+          return fail(QCS_HIP_EINVAL, "pass %d: gate %d: bad control", p, g);
+      }
+    } else {
+      if (ps.first_block + ps.num_blocks > nb)
+        return fail(QCS_HIP_EINVAL, "pass %d: block range outside the block array", p);
+      int rc = validate_blocks(n, blocks + ps.first_block, ps.num_blocks, gates, ng);
+      if (rc) return rc;
+      for (int bi = ps.first_block; bi < ps.first_block + ps.num_blocks; ++bi) {
+        if (blocks[bi].k > tb)
+          return fail(QCS_HIP_EINVAL, "block %d: k > tile_bits", bi);
+        for (int q = 0; q < blocks[bi].k; ++q)
+          if (!((mask >> blocks[bi].targets[q]) & 1))
+            return fail(QCS_HIP_EINVAL, "block %d: target outside the pass bits", bi);
+      }
+    }
+  }
+  return 0;
+}
+
 static int validate_batch(const qcs_hip_batch_circuit *cs, int nc,
                           const qcs_hip_pass *passes, int np,
                           const qcs_hip_block *blocks, int nb,
@@ -1132,65 +1215,32 @@ static int validate_batch(const qcs_hip_batch_circuit *cs, int nc,
     return fail(QCS_HIP_EINVAL, "null uniforms or output");
   if (shots > (long(1) << 40) / (nc > 0 ? nc : 1))
     return fail(QCS_HIP_EINVAL, "too many shots");
-  for (int i = 0; i < nc; ++i) {
-    const qcs_hip_batch_circuit &c = cs[i];
-    if (c.num_qubits < 1 || c.num_qubits > kBMaxN)
-      return fail(QCS_HIP_EINVAL, "circuit %d: num_qubits %d outside [1,%d]", i,
-                  c.num_qubits, kBMaxN);
-    if (c.precision != QCS_HIP_C128 && c.precision != QCS_HIP_C64)
-      return fail(QCS_HIP_EINVAL, "circuit %d: unknown precision %d", i, c.precision);
-    const int n = c.num_qubits, tb = c.tile_bits;
-    if (tb < 1 || tb > n || tb > max_tb(c.precision))
-      return fail(QCS_HIP_EINVAL, "circuit %d: tile_bits %d outside [1,%d]", i, tb,
-                  std::min(n, max_tb(c.precision)));
-    if (c.precision == QCS_HIP_C64 && tb < 5 && tb != n)
-      return fail(QCS_HIP_EINVAL, "circuit %d: complex64 tiles of < 5 bits must hold the state", i);
-    if (c.num_passes < 1 || c.first_pass < 0 || c.first_pass + c.num_passes > np)
-      return fail(QCS_HIP_EINVAL, "circuit %d: pass range outside the pass array", i);
-    for (int p = c.first_pass; p < c.first_pass + c.num_passes; ++p) {
-      const qcs_hip_pass &ps = passes[p];
-      if (ps.tile_bits != tb)
-        return fail(QCS_HIP_EINVAL, "pass %d: tile_bits %d != circuit's %d", p,
-                    ps.tile_bits, tb);
-      for (int j = 0; j < tb; ++j) {
-        if (ps.bits[j] < 0 || ps.bits[j] >= n || (j > 0 && ps.bits[j] <= ps.bits[j - 1]))
-          return fail(QCS_HIP_EINVAL, "pass %d: bits not ascending in range", p);
-      }
-      if (c.precision == QCS_HIP_C64 && ps.bits[0] != 0)
-        return fail(QCS_HIP_EINVAL, "pass %d: complex64 tiles must hold bit 0", p);
-      unsigned long long mask = 0;
-      for (int j = 0; j < tb; ++j) mask |= 1ull << ps.bits[j];
-      if (ps.first_block < 0 || ps.num_blocks < 0)
-        return fail(QCS_HIP_EINVAL, "pass %d: bad op range", p);
-      if (c.precision == QCS_HIP_C128) {
-        if (ps.first_block + ps.num_blocks > ng)
-          return fail(QCS_HIP_EINVAL, "pass %d: gate range outside the gate array", p);
-        for (int g = ps.first_block; g < ps.first_block + ps.num_blocks; ++g) {
-          const qcs_hip_gate &gt = gates[g];
-          if (gt.kind < QCS_HIP_G_1Q || gt.kind > QCS_HIP_G_SWAP || gt.target < 0 ||
-              gt.target >= n || !((mask >> gt.target) & 1))
-            return fail(QCS_HIP_EINVAL, "pass %d: gate %d outside the pass bits", p, g);
-          if (gt.kind != QCS_HIP_G_1Q &&
-              (gt.control < 0 || gt.control >= n || gt.control == gt.target ||
-               !((mask >> gt.control) & 1)))
-            return fail(QCS_HIP_EINVAL, "pass %d: gate %d: bad control", p, g);
+  // circuits in chunks on host threads; the first failing circuit (in
+  // order) is checked again on this thread, whose message the caller reads
+  constexpr int kChunkCircuits = 32;
+  const int nt = std::max(1, std::min(nc / kChunkCircuits, host_threads()));
+  std::atomic<int> next{0}, first_bad{nc};
+  auto work = [&]() {
+    for (;;) {
+      const int c0 = next.fetch_add(kChunkCircuits);
+      if (c0 >= nc || c0 > first_bad.load()) break;
+      for (int i = c0; i < std::min(nc, c0 + kChunkCircuits); ++i)
+        if (validate_circuit(i, cs, passes, np, blocks, nb, gates, ng)) {
+          int cur = first_bad.load();
+          while (i < cur && !first_bad.compare_exchange_weak(cur, i)) {
+          }
+          break;
         }
-      } else {
-        if (ps.first_block + ps.num_blocks > nb)
-          return fail(QCS_HIP_EINVAL, "pass %d: block range outside the block array", p);
-        int rc = validate_blocks(n, blocks + ps.first_block, ps.num_blocks, gates, ng);
-        if (rc) return rc;
-        for (int bi = ps.first_block; bi < ps.first_block + ps.num_blocks; ++bi) {
-          if (blocks[bi].k > tb)
-            return fail(QCS_HIP_EINVAL, "block %d: k > tile_bits", bi);
-          for (int q = 0; q < blocks[bi].k; ++q)
-            if (!((mask >> blocks[bi].targets[q]) & 1))
-              return fail(QCS_HIP_EINVAL, "block %d: target outside the pass bits", bi);
-        }
-      }
     }
+  };
+  {
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; ++t) th.emplace_back(work);
+    work();
+    for (auto &x : th) x.join();
   }
-  return 0;
+  const int bad = first_bad.load();
+  return bad < nc ? validate_circuit(bad, cs, passes, np, blocks, nb, gates, ng) : 0;
 }
 
 static int position_in(const qcs_hip_pass &ps, int q) {
@@ -1512,13 +1562,6 @@ static int submit(qcs_hip_batch *b, bool rerun, int nb, bool build_graph) {
 
 // host threads for building descriptors: QCS_HOST_THREADS, else the
 // hardware's, at most 64
-static int host_threads() {
-  int t = int(std::thread::hardware_concurrency());
-  const char *env = std::getenv("QCS_HOST_THREADS");
-  if (env && std::atoi(env) > 0) t = std::atoi(env);
-  return std::max(1, std::min(t, 64));
-}
-
 static double host_ms() {
   return std::chrono::duration<double, std::milli>(
              std::chrono::steady_clock::now().time_since_epoch())
@@ -1551,16 +1594,19 @@ int batch_run(qcs_hip_batch *b, const qcs_hip_batch_circuit *cs, int nc,
   // device descriptors: passes, then their gates / register groups / blocks,
   // built for contiguous ranges of circuits on host threads and joined
   std::vector<BPass> hp(static_cast<size_t>(np));
-  struct Part {
-    std::vector<BGate> g;
-    std::vector<BGroup> grp;
-    std::vector<BBlock> blk;
-    std::vector<int> own;  // passes built by this part
-  };
   const int nparts = std::max(1, std::min(nc / 64 + 1, host_threads()));
-  std::vector<Part> parts(static_cast<size_t>(nparts));
+  // the parts' op lists live in the batch between runs, so their memory is
+  // reused rather than faulted in again on every planned run
+  std::vector<BatchPart> &parts = b->parts;
+  if (parts.size() < size_t(nparts)) parts.resize(size_t(nparts));
+  for (int t = 0; t < nparts; ++t) {
+    parts[size_t(t)].g.clear();
+    parts[size_t(t)].grp.clear();
+    parts[size_t(t)].blk.clear();
+    parts[size_t(t)].own.clear();
+  }
   auto build_part = [&](int t) {
-    Part &pt = parts[size_t(t)];
+    BatchPart &pt = parts[size_t(t)];
     const int i0 = int(int64_t(nc) * t / nparts), i1 = int(int64_t(nc) * (t + 1) / nparts);
     for (int i = i0; i < i1; ++i) {
       const qcs_hip_batch_circuit &c = cs[i];
@@ -1754,7 +1800,7 @@ int batch_run(qcs_hip_batch *b, const qcs_hip_batch_circuit *cs, int nc,
   // the op indices of the passes it owns), and a share of the composer's
   // gates; parts own disjoint passes and ranges
   auto fill_part = [&](int t) {
-    Part &pt = parts[size_t(t)];
+    BatchPart &pt = parts[size_t(t)];
     const int o_g = int(og[size_t(t)]), o_grp = int(ogrp[size_t(t)]), o_b = int(ob[size_t(t)]);
     for (int p : pt.own) {
       const int i = owner[size_t(p)];

@@ -42,7 +42,26 @@ struct s_q_batch {
   t_q_circuit **last_circ;
   long *last_version;
   int last_count, last_cap, last_shots, last_valid;
+  /* the concatenated plan and the uniforms of a run, kept between runs so
+     their pages are not faulted in again every time */
+  qcs_hip_pass *cat_pass;
+  qcs_hip_block *cat_block;
+  qcs_hip_gate *cat_gate;
+  double *cat_u;
+  size_t cat_pass_cap, cat_block_cap, cat_gate_cap, cat_u_cap;
 };
+
+/* *p holds at least `need` elements of `size` bytes afterwards (NULL if out
+   of memory, the old buffer freed); its contents are not kept */
+static void *reserve(void **p, size_t *cap, size_t need, size_t size) {
+  if (need < 1) need = 1;
+  if (*p && *cap >= need) return *p;
+  free(*p);
+  *cap = 0;
+  *p = malloc(need * size);
+  if (*p) *cap = need;
+  return *p;
+}
 
 t_q_batch *qc_batch_create(int options) {
   t_q_batch *b;
@@ -69,6 +88,10 @@ void qc_batch_destroy(t_q_batch *batch) {
   qcs_hip_batch_destroy(batch->dev);
   free(batch->last_circ);
   free(batch->last_version);
+  free(batch->cat_pass);
+  free(batch->cat_block);
+  free(batch->cat_gate);
+  free(batch->cat_u);
   free(batch);
 }
 
@@ -330,6 +353,21 @@ static void concat_all(concat_job *job) {
   }
 }
 
+/* the shots' uniforms, rand() / RAND_MAX in the order qc_sample on each
+   circuit in turn would draw them; drawn on a thread of their own while the
+   circuits are planned (the planners call no rand()) */
+typedef struct {
+  double *u;
+  long total;
+} batch_uniform_job;
+
+static void *draw_batch_uniforms(void *arg) {
+  const batch_uniform_job *job = (const batch_uniform_job *)arg;
+  long s;
+  for (s = 0; s < job->total; s++) job->u[s] = rand() / (double)RAND_MAX;
+  return NULL;
+}
+
 static void plan_all(const t_q_batch *b, t_q_circuit **circuits, int count,
                      circuit_plan *plans) {
   plan_job job;
@@ -359,8 +397,10 @@ int qc_batch_sample(t_q_batch *batch, t_q_circuit **circuits, int count,
   concat_job job;
   double *u = NULL;
   long total, s, np = 0, nb = 0, ng = 0;
-  int i, rc = QCS_HIP_ENOMEM;
+  int i, rc = QCS_HIP_ENOMEM, drawing = 0;
   double t0, t1, t2, t3;
+  batch_uniform_job uj;
+  pthread_t uth;
   if (!batch || count < 0 || shots < 0 || (count > 0 && !circuits) ||
       (count > 0 && shots > 0 && !indices))
     return QCS_HIP_EINVAL;
@@ -374,7 +414,8 @@ int qc_batch_sample(t_q_batch *batch, t_q_circuit **circuits, int count,
   total = (long)count * (long)shots;
   if (same_as_last(batch, circuits, count, shots)) {
     if (total > 0) {
-      u = (double *)malloc((size_t)total * sizeof(double));
+      u = (double *)reserve((void **)&batch->cat_u, &batch->cat_u_cap, (size_t)total,
+                            sizeof(double));
       if (!u) return QCS_HIP_ENOMEM;
     }
     for (s = 0; s < total; s++) u[s] = rand() / (double)RAND_MAX;
@@ -387,16 +428,20 @@ int qc_batch_sample(t_q_batch *batch, t_q_circuit **circuits, int count,
       batch->last_valid = 0;
       fprintf(stderr, "Error: batch run failed: %s\n", qcs_hip_last_error());
     }
-    free(u);
     return rc;
   }
   batch->last_valid = 0;
   plans = (circuit_plan *)calloc((size_t)count, sizeof(circuit_plan));
   circ = (qcs_hip_batch_circuit *)calloc((size_t)count, sizeof(qcs_hip_batch_circuit));
   offs = (long *)malloc(3 * (size_t)count * sizeof(long));
-  if (total > 0) u = (double *)malloc((size_t)total * sizeof(double));
+  if (total > 0)
+    u = (double *)reserve((void **)&batch->cat_u, &batch->cat_u_cap, (size_t)total,
+                          sizeof(double));
   if (!plans || !circ || !offs || (total > 0 && !u)) goto out;
   t0 = now_ms();
+  uj.u = u;
+  uj.total = total;
+  drawing = total > 0 && pthread_create(&uth, NULL, draw_batch_uniforms, &uj) == 0;
   plan_all(batch, circuits, count, plans);
   t1 = now_ms();
   for (i = 0; i < count; i++) {
@@ -415,9 +460,12 @@ int qc_batch_sample(t_q_batch *batch, t_q_circuit **circuits, int count,
     rc = QCS_HIP_EINVAL;
     goto out;
   }
-  pass = (qcs_hip_pass *)malloc((size_t)np * sizeof(qcs_hip_pass));
-  block = (qcs_hip_block *)malloc((size_t)(nb > 0 ? nb : 1) * sizeof(qcs_hip_block));
-  gate = (qcs_hip_gate *)malloc((size_t)(ng > 0 ? ng : 1) * sizeof(qcs_hip_gate));
+  pass = (qcs_hip_pass *)reserve((void **)&batch->cat_pass, &batch->cat_pass_cap,
+                                 (size_t)np, sizeof(qcs_hip_pass));
+  block = (qcs_hip_block *)reserve((void **)&batch->cat_block, &batch->cat_block_cap,
+                                   (size_t)nb, sizeof(qcs_hip_block));
+  gate = (qcs_hip_gate *)reserve((void **)&batch->cat_gate, &batch->cat_gate_cap,
+                                 (size_t)ng, sizeof(qcs_hip_gate));
   if (!pass || !block || !gate) goto out;
   /* concatenate on host threads: circuit i's ranges start at its offsets */
   job.circuits = circuits;
@@ -432,28 +480,31 @@ int qc_batch_sample(t_q_batch *batch, t_q_circuit **circuits, int count,
   job.count = count;
   concat_all(&job);
   t2 = now_ms();
-  for (s = 0; s < total; s++) u[s] = rand() / (double)RAND_MAX;
+  if (drawing) {
+    pthread_join(uth, NULL);
+    drawing = 0;
+  } else {
+    draw_batch_uniforms(&uj);
+  }
   t3 = now_ms();
   rc = qcs_hip_batch_run(batch->dev, circ, count, pass, (int)np, block, (int)nb,
                          gate, (int)ng, shots, u, indices);
   if (profiling())
     fprintf(stderr, "qc_batch_sample: plan %.1f ms (%d threads), concatenate %.1f ms, "
-                    "uniforms %.1f ms, run %.1f ms\n", t1 - t0, host_threads(count),
+                    "uniforms (the rest of their draw) %.1f ms, run %.1f ms\n", t1 - t0,
+            host_threads(count),
             t2 - t1, t3 - t2, now_ms() - t3);
   if (rc != 0)
     fprintf(stderr, "Error: batch run failed: %s\n", qcs_hip_last_error());
   else
     remember(batch, circuits, count, shots);
 out:
+  if (drawing) pthread_join(uth, NULL);
   if (plans)
     for (i = 0; i < count; i++) free_plan(&plans[i]);
   free(plans);
   free(offs);
   free(circ);
-  free(pass);
-  free(block);
-  free(gate);
-  free(u);
   return rc;
 }
 
