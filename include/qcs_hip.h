@@ -208,7 +208,10 @@ int qcs_hip_apply_unitary(qcs_hip_state *s, int k, const int *targets,
  * states with num_qubits >= tile_bits + 3; tile_bits is 13; blocks of a
  * pass act on at most QCS_HIP_TILE_KMAX qubits and a pass holds at most
  * QCS_HIP_TILE_MAX_BLOCKS blocks (their operands stay in LDS;
- * QCS_HIP_TILE_MAX_BLOCKS_SPLIT with QCS_HIP_TILE_MATH_SPLIT_BF16).  No
+ * QCS_HIP_TILE_MAX_BLOCKS_SPLIT with QCS_HIP_TILE_MATH_SPLIT_BF16).  The
+ * call returns once the passes are queued on the state's stream (the plan
+ * arrays may be freed at once); a fault in them is reported by the next call
+ * that waits for the stream (qcs_hip_sync, reads, the next plan).  No
  * reference counterpart. */
 #define QCS_HIP_TILE_BITS 13
 #define QCS_HIP_TILE_LOW_BITS 4

@@ -1207,8 +1207,11 @@ __global__ __launch_bounds__(NTH) void k_tile_program(
 
 // ---- host side ------------------------------------------------------------------
 
-static int grow_dev(void **p, size_t *cap, size_t need) {
+// a device buffer of at least `need` bytes; the old one is freed only once
+// the state's stream has drained (a previous call's passes may still read it)
+static int grow_dev(qcs_hip_state *s, void **p, size_t *cap, size_t need) {
   if (*cap >= need) return 0;
+  QCS_HIP_TRY(hipStreamSynchronize(s->stream));
   if (*p) (void)hipFree(*p);
   *p = nullptr;
   *cap = 0;
@@ -1613,8 +1616,8 @@ int tiles_apply(qcs_hip_state *s, const qcs_hip_pass *passes, int np,
   const size_t pbytes = sizeof(TilePass) * size_t(np);
   const size_t tbytes = sizeof(TileBlock) * size_t(nb > 0 ? nb : 1);
   int rc;
-  if ((rc = grow_dev(&s->tpass_dev, &s->tpass_cap, pbytes))) return rc;
-  if ((rc = grow_dev(&s->tblock_dev, &s->tblock_cap, tbytes))) return rc;
+  if ((rc = grow_dev(s, &s->tpass_dev, &s->tpass_cap, pbytes))) return rc;
+  if ((rc = grow_dev(s, &s->tblock_dev, &s->tblock_cap, tbytes))) return rc;
   const int nwb = s->tile_threads == 1024 ? 4 : 3;  // wave-index bits
   const bool split_math = s->tile_math == QCS_HIP_TILE_MATH_SPLIT_BF16;
   // k = 4 blocks as pipelined 16-column halves: FP32 math, 1024 threads;
@@ -1717,8 +1720,11 @@ int tiles_apply(qcs_hip_state *s, const qcs_hip_pass *passes, int np,
         (rc = fold_norm_partials(s, parts, &ns_dev)))
       return rc;
   }
-  // the staging buffer holds the descriptors until the copies have run
-  QCS_HIP_TRY(hipStreamSynchronize(s->stream));
+  // No wait for the passes: the pinned staging buffer that the descriptor
+  // copies read is written again only after a stream synchronisation (every
+  // host writer of it waits first), and the device buffers only after one
+  // (grow_dev, compose_blocks), so the caller prepares its next work while
+  // the passes run.  A fault in them surfaces at the next synchronisation.
   return 0;
 }
 
