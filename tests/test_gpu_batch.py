@@ -323,3 +323,56 @@ def test_c128_fused_rotations_and_folded_permutations(qlib, cl, n):
     assert np.array_equal(amps[0], ref_a)
     assert np.array_equal(idx[0], ref_i)
     b.close()
+
+
+def test_batch_run_reports_first_invalid_circuit(qlib):
+    """qcs_hip_batch_run checks the circuits on host threads in chunks; the
+    error it reports is still the first failing circuit in order (checked
+    again on the calling thread for the message), whichever chunk finishes
+    first.  No device work runs for a rejected batch."""
+    import ctypes
+    hip, _ = qlib
+    nc = 200
+    h = 2 ** -0.5
+    G = (qcs_amd.HipGate * nc)()
+    P = (qcs_amd.HipPass * nc)()
+    for i in range(nc):
+        G[i].kind, G[i].target, G[i].control = qcs_amd._native.G_1Q, i % 4, -1
+        for j, v in enumerate((h, 0, h, 0, h, 0, -h, 0)):
+            G[i].m[j] = v
+        P[i].tile_bits, P[i].first_block, P[i].num_blocks = 4, i, 1
+        for j in range(4):
+            P[i].bits[j] = j
+
+    def circuits():
+        cs = (qcs_amd.HipBatchCircuit * nc)()
+        for i in range(nc):
+            cs[i].num_qubits, cs[i].precision, cs[i].tile_bits = 4, qcs_amd._native.C128, 4
+            cs[i].first_pass, cs[i].num_passes = i, 1
+        return cs
+
+    b = ctypes.c_void_p()
+    assert hip.qcs_hip_batch_create(ctypes.byref(b), 0) == 0
+    try:
+        u = (ctypes.c_double * nc)(*([0.5] * nc))
+        out = (ctypes.c_long * nc)()
+
+        def run(cs):
+            return hip.qcs_hip_batch_run(b, cs, nc, P, nc, None, 0, G, nc, 1, u, out)
+
+        assert run(circuits()) == 0  # the valid batch runs
+        cs = circuits()
+        cs[170].precision = 7
+        assert run(cs) == -1
+        assert b"circuit 170: unknown precision" in hip.qcs_hip_last_error()
+        cs[150].num_qubits = 0
+        assert run(cs) == -1
+        assert b"circuit 150: num_qubits 0" in hip.qcs_hip_last_error()
+        cs = circuits()
+        G[190].target = 9  # outside circuit 190's 4 qubits and its pass bits
+        assert run(cs) == -1
+        assert b"pass 190: gate 190 outside the pass bits" in hip.qcs_hip_last_error()
+        G[190].target = 1
+        assert run(circuits()) == 0
+    finally:
+        hip.qcs_hip_batch_destroy(b)
