@@ -280,7 +280,7 @@ def test_tile_direct_passes_bit_identical(qlib, n, seed):
                                       ctypes.byref(v)) == 0 and v.value == 1
 
 
-@pytest.mark.parametrize("n,seed", [(16, 4), (20, 5), (22, 6)])
+@pytest.mark.parametrize("n,seed", [(16, 4), (20, 5), (22, 6), (24, 7)])
 def test_apply_passes_normalize_equals_apply_then_normalize(qlib, n, seed):
     """qcs_hip_apply_passes_normalize (three-product form: the pass before
     the last sums |x|^2 of what it writes and the last pass scales its first
@@ -289,8 +289,9 @@ def test_apply_passes_normalize_equals_apply_then_normalize(qlib, n, seed):
     change of one FP32 pass (~1e-7), so the states agree to 1e-6, and a state
     of norm^2 2.89 comes out of norm 1 within 1e-5.  Covered: the last passes
     direct or through LDS, a one-pass plan (the last pass sums the norm, the
-    rescale follows), the chain form (the separate norm and rescale), a zero
-    state (left as it is, q_state_normalize's norm_sq > 1e-12 rule)."""
+    rescale follows), a grid with more work-groups than the reduction buffer
+    has partial slots and the chain form (the separate norm and rescale), a
+    zero state (left as it is, q_state_normalize's norm_sq > 1e-12 rule)."""
     from test_fusion_plan import pass_plan
     hip, _ = qlib
     print(f"[fused] apply+normalize n={n}", flush=True)
@@ -298,10 +299,16 @@ def test_apply_passes_normalize_equals_apply_then_normalize(qlib, n, seed):
     passes, blocks, ordered = pass_plan(o, n)
     assert len(passes) >= 2
     psi = 1.7 * rand_state(n, np.random.default_rng(5 * n + seed))
-    for form, direct, npass in ((qcs_amd.TILE_K4_GAUSS, 1, len(passes)),
-                                (qcs_amd.TILE_K4_GAUSS, 0, len(passes)),
-                                (qcs_amd.TILE_K4_GAUSS, 1, 1),
-                                (qcs_amd.TILE_K4_CHAIN, 1, len(passes))):
+    cases = [(qcs_amd.TILE_K4_GAUSS, 1, len(passes), 0),
+             (qcs_amd.TILE_K4_GAUSS, 0, len(passes), 0),
+             (qcs_amd.TILE_K4_GAUSS, 1, 1, 0),
+             (qcs_amd.TILE_K4_CHAIN, 1, len(passes), 0)]
+    if (1 << (n - 13)) > 1024:
+        # grid 1024 launches 2048 work-groups on direct passes (two fit a
+        # CU), more than the reduction buffer's 1024 partial slots: the
+        # separate norm pass runs
+        cases.append((qcs_amd.TILE_K4_GAUSS, 1, len(passes), 1024))
+    for form, direct, npass, grid in cases:
         # a one-pass plan: the first pass's blocks only
         nblk = sum(p.num_blocks for p in passes[:npass])
         ngate = sum(b.num_gates for b in blocks[:nblk])
@@ -316,6 +323,9 @@ def test_apply_passes_normalize_equals_apply_then_normalize(qlib, n, seed):
                                               form) == 0
                 assert hip.qcs_hip_set_tuning(st.device_state, qcs_amd.TUNE_TILE_DIRECT,
                                               direct) == 0
+                if grid:
+                    assert hip.qcs_hip_set_tuning(st.device_state, qcs_amd.TUNE_TILE_GRID,
+                                                  grid) == 0
                 if fused:
                     rc = hip.qcs_hip_apply_passes_normalize(st.device_state, P, npass,
                                                             B, nblk, G, ngate)
@@ -325,8 +335,8 @@ def test_apply_passes_normalize_equals_apply_then_normalize(qlib, n, seed):
                     rc = rc or hip.qcs_hip_normalize(st.device_state, None)
                 assert rc == 0, hip.qcs_hip_last_error()
                 got.append(st.amplitudes())
-        assert abs(np.vdot(got[0], got[0]).real - 1.0) <= 1e-5, (form, direct, npass)
-        assert np.linalg.norm(got[0] - got[1]) <= 1e-6, (form, direct, npass)
+        assert abs(np.vdot(got[0], got[0]).real - 1.0) <= 1e-5, (form, direct, npass, grid)
+        assert np.linalg.norm(got[0] - got[1]) <= 1e-6, (form, direct, npass, grid)
     P = (qcs_amd.HipPass * len(passes))(*passes)
     B = (qcs_amd.HipBlock * len(blocks))(*blocks)
     G = (qcs_amd.HipGate * len(ordered))(*ordered)
